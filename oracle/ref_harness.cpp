@@ -1,0 +1,294 @@
+// Harness that exposes the compiled reference (its own sources, built in
+// place by oracle/Makefile into oracle/_ref/libmcrt_ref.so) through a small
+// C API, so the tests can pin the oracle and the product against the real
+// reference code.  TEST INFRASTRUCTURE ONLY.
+//
+// Everything here calls the reference's public classes; nothing restates
+// its algorithms.  The SplitBVH root is built with the same steps as
+// SplitBVH::SplitBVH (splitbvh.cpp:1410-1433) through the public static
+// SplitBVH::build, because the root pointer itself is private.
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include <glm/glm.hpp>
+
+#include "aabb.h"
+#include "boundedtriangle.h"
+#include "camera.h"
+#include "constants.h"
+#include "debug_mesh.h"
+#include "material.h"
+#include "mesh.h"
+#include "ray.h"
+#include "sample.h"
+#include "scenebvh.h"
+#include "splitbvh.h"
+#include "triangle.h"
+
+#define REF_API extern "C" __attribute__((visibility("default")))
+
+namespace
+{
+struct RefBVH
+{
+    std::vector<std::shared_ptr<Triangle>> tris;
+    std::shared_ptr<SplitBVHNode> root;
+};
+
+struct Fnv
+{
+    uint64_t h = 1469598103934665603ull;
+    void bytes(const void* p, size_t n)
+    {
+        const unsigned char* c = (const unsigned char*)p;
+        for (size_t i = 0; i < n; i++) { h ^= c[i]; h *= 1099511628211ull; }
+    }
+    void f(float x) { if (x == 0.0f) x = 0.0f; bytes(&x, 4); }
+    void i(int32_t x) { bytes(&x, 4); }
+    void box(const AABB& b)
+    {
+        f(b.min().x); f(b.min().y); f(b.min().z); f(b.max().x); f(b.max().y); f(b.max().z);
+    }
+};
+
+struct Walk
+{
+    long long nodes = 0, leaves = 0, refs = 0, empty = 0;
+    int max_depth = 0;
+};
+
+void digest(const std::shared_ptr<SplitBVHNode>& n, Fnv& h, Walk& w, int depth)
+{
+    w.nodes++;
+    if (depth > w.max_depth) w.max_depth = depth;
+    auto leaf = std::dynamic_pointer_cast<SplitBVHLeafNode>(n);
+    if (leaf)
+    {
+        w.leaves++;
+        int cnt = 0;
+        for (int i = 0; i < k_sbvh_leaf_node_size; i++) if (leaf->primitive[i]) cnt++;
+        if (cnt == 0) w.empty++;
+        w.refs += cnt;
+        h.i(1);
+        h.box(*leaf->bounding_box);
+        h.i(cnt);
+        for (int i = 0; i < k_sbvh_leaf_node_size; i++)
+        {
+            if (!leaf->primitive[i]) continue;
+            h.i(leaf->primitive[i]->originalTriangle()->primitiveID());
+            h.box(leaf->primitive[i]->clippedBoundingBox());
+        }
+        return;
+    }
+    auto br = std::dynamic_pointer_cast<SplitBVHBranchNode>(n);
+    h.i(0);
+    h.box(*br->bounding_box);
+    h.i(br->split_axis);
+    digest(br->left_child, h, w, depth + 1);
+    digest(br->right_child, h, w, depth + 1);
+}
+
+vec3 V(const float* p) { return vec3(p[0], p[1], p[2]); }
+}  // namespace
+
+REF_API void* ref_bvh_build(const float* tris9, int n)
+{
+    RefBVH* b = new RefBVH();
+    std::vector<std::shared_ptr<BoundedTriangle>> refs;
+    for (int i = 0; i < n; i++)
+    {
+        auto t = std::make_shared<Triangle>(V(tris9 + 9 * i), V(tris9 + 9 * i + 3), V(tris9 + 9 * i + 6));
+        t->setPrimitiveID(i);
+        t->setMaterialID(0);
+        b->tris.push_back(t);
+    }
+    for (const auto& t : b->tris) refs.push_back(std::make_shared<BoundedTriangle>(t, t->boundingBox()));
+    b->root = SplitBVH::build(refs, 0u, "root");
+    return b;
+}
+
+REF_API void ref_bvh_free(void* h) { delete (RefBVH*)h; }
+
+// stats: [nodes, leaves, refs, max_depth, empty_leaves]
+REF_API uint64_t ref_bvh_digest(void* h, long long* stats)
+{
+    RefBVH* b = (RefBVH*)h;
+    Fnv f;
+    Walk w;
+    digest(b->root, f, w, 0);
+    if (stats)
+    {
+        stats[0] = w.nodes; stats[1] = w.leaves; stats[2] = w.refs; stats[3] = w.max_depth;
+        stats[4] = w.empty;
+    }
+    return f.h;
+}
+
+// rays n x 8 (ox oy oz tmin dx dy dz tmax) -> SplitBVH::intersect semantics.
+REF_API void ref_bvh_intersect(void* h, const float* rays, long long n, int* prim, float* tuv)
+{
+    RefBVH* b = (RefBVH*)h;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (long long i = 0; i < n; i++)
+    {
+        const float* r = rays + 8 * i;
+        Ray ray(V(r), V(r + 4), r[3], r[7]);
+        HitInfo hit;
+        hit.reset();
+        bool ok = b->root->intersect(ray, hit);
+        prim[i] = ok ? hit.m_objID : -1;
+        tuv[3 * i] = ok ? hit.m_t : INFINITY;
+        tuv[3 * i + 1] = ok ? hit.m_u : 0.0f;
+        tuv[3 * i + 2] = ok ? hit.m_v : 0.0f;
+    }
+}
+
+// The plug-in boundary itself: SceneBVH over TriangleObjects (scenebvh.cpp:340-445).
+// rays n x 6 (o, d); out: prim, t/u/v, shading normal n x 3; occluded with tfar.
+REF_API void ref_scenebvh_query(const float* tris9, int ntri, const float* rays6, long long n,
+                                float tfar, int* prim, float* tuv, float* nrm, uint8_t* occ)
+{
+    std::vector<TriangleObject> tris;
+    for (int i = 0; i < ntri; i++)
+    {
+        TriangleObject t(V(tris9 + 9 * i), V(tris9 + 9 * i + 3), V(tris9 + 9 * i + 6));
+        t.setMaterialID(0);
+        tris.push_back(t);
+    }
+    SceneBVH bvh(tris);
+    for (long long i = 0; i < n; i++)
+    {
+        Ray r(V(rays6 + 6 * i), V(rays6 + 6 * i + 3));
+        HitInfo hh = bvh.intersect(r);
+        prim[i] = hh.primitiveID();
+        tuv[3 * i] = hh.m_t; tuv[3 * i + 1] = hh.m_u; tuv[3 * i + 2] = hh.m_v;
+        nrm[3 * i] = hh.m_nl.x; nrm[3 * i + 1] = hh.m_nl.y; nrm[3 * i + 2] = hh.m_nl.z;
+        occ[i] = bvh.occluded(r, tfar) ? 1 : 0;
+    }
+}
+
+REF_API int ref_debug_mesh(float* tris9)
+{
+    DebugMesh m;
+    const auto& t = m.triangles();
+    if (tris9)
+        for (size_t i = 0; i < t.size(); i++)
+        {
+            const vec3* p[3] = {&t[i].a, &t[i].b, &t[i].c};
+            for (int k = 0; k < 3; k++)
+            {
+                tris9[9 * i + 3 * k] = p[k]->x;
+                tris9[9 * i + 3 * k + 1] = p[k]->y;
+                tris9[9 * i + 3 * k + 2] = p[k]->z;
+            }
+        }
+    return (int)t.size();
+}
+
+REF_API int ref_ray_box(const float* ray8, const float* box6)
+{
+    Ray r(V(ray8), V(ray8 + 4), ray8[3], ray8[7]);
+    AABB b(V(box6), V(box6 + 3));
+    return b.overlap(r) ? 1 : 0;
+}
+
+REF_API int ref_ray_tri(const float* ray8, const float* tri9, float* tuv)
+{
+    Ray r(V(ray8), V(ray8 + 4), ray8[3], ray8[7]);
+    TriangleObject t(V(tri9), V(tri9 + 3), V(tri9 + 6));
+    t.setMaterialID(0);
+    t.setPrimitiveID(0);
+    HitInfo h;
+    bool ok = t.intersect(r, h);
+    if (ok) { tuv[0] = h.m_t; tuv[1] = h.m_u; tuv[2] = h.m_v; }
+    return ok ? 1 : 0;
+}
+
+REF_API void ref_clip_bound_tight(const float* tri9, const float* box6, float lo, float hi,
+                                  int axis, float* out6)
+{
+    auto t = std::make_shared<Triangle>(V(tri9), V(tri9 + 3), V(tri9 + 6));
+    BoundedTriangle bt(t, AABB(V(box6), V(box6 + 3)));
+    AABB r = bt.clipBoundTight(lo, hi, axis);
+    out6[0] = r.min().x; out6[1] = r.min().y; out6[2] = r.min().z;
+    out6[3] = r.max().x; out6[4] = r.max().y; out6[5] = r.max().z;
+}
+
+// Mesh + genNormals (mesh.cpp:24-118) and the loadModel vertex transform
+// (scene.cpp:374-381) with the given column-major matrix (NULL = identity).
+REF_API long long ref_load_obj(const char* path, const float* m16, float* tris9, float* vn9)
+{
+    Mesh obj(path);
+    long long nf = (long long)obj.m_faces.size();
+    if (!tris9) return nf;
+    if (m16)
+    {
+        glm::mat4 m;
+        for (int c = 0; c < 4; c++)
+            for (int r = 0; r < 4; r++) m[c][r] = m16[4 * c + r];
+        for (size_t k = 0; k < obj.m_verts.size(); k++)
+        {
+            glm::vec4 v = glm::vec4(obj.m_verts[k].x, obj.m_verts[k].y, obj.m_verts[k].z, 1.0f);
+            v = m * v;
+            obj.m_verts[k] = vec3(v.x, v.y, v.z);
+        }
+        obj.genNormals();
+    }
+    for (long long i = 0; i < nf; i++)
+    {
+        int idx[3] = {obj.m_faces[i].x, obj.m_faces[i].y, obj.m_faces[i].z};
+        for (int k = 0; k < 3; k++)
+        {
+            tris9[9 * i + 3 * k] = obj.m_verts[idx[k]].x;
+            tris9[9 * i + 3 * k + 1] = obj.m_verts[idx[k]].y;
+            tris9[9 * i + 3 * k + 2] = obj.m_verts[idx[k]].z;
+            vn9[9 * i + 3 * k] = obj.m_vertex_normals[idx[k]].x;
+            vn9[9 * i + 3 * k + 1] = obj.m_vertex_normals[idx[k]].y;
+            vn9[9 * i + 3 * k + 2] = obj.m_vertex_normals[idx[k]].z;
+        }
+    }
+    return nf;
+}
+
+// Camera::makeRay at (mx, my) = pxy / (w, h); rays6 n x 6.
+REF_API void ref_camera_rays(const float* o, const float* at, int w, int h, float fov,
+                             const float* pxy, long long n, float* rays6)
+{
+    Camera cam(V(o), V(at), vec3(0, 1, 0), w, h, fov);
+    for (long long i = 0; i < n; i++)
+    {
+        Ray r = cam.makeRay(pxy[2 * i] / w, pxy[2 * i + 1] / h);
+        rays6[6 * i] = r.orig.x; rays6[6 * i + 1] = r.orig.y; rays6[6 * i + 2] = r.orig.z;
+        rays6[6 * i + 3] = r.dir.x; rays6[6 * i + 4] = r.dir.y; rays6[6 * i + 5] = r.dir.z;
+    }
+}
+
+// Material::sample / pdfW / evaluate for type (1 DIFF, 4 GLSY) with colour
+// rgb: in n x (wi3, nl3, r1, r2); out n x (wo3, weight3, pdf_of_wo, eval3).
+REF_API void ref_bsdf(int type, const float* rgb, const float* in8, long long n, float* out10)
+{
+    Material m(V(rgb), 0.0f, (MaterialType)type);
+    for (long long i = 0; i < n; i++)
+    {
+        const float* q = in8 + 8 * i;
+        vec3 wi = V(q), nl = V(q + 3), w;
+        vec3 wo = m.sample(wi, nl, q[6], q[7], w);
+        float pdf = m.pdfW(wi, nl, wo);
+        vec3 ev = m.evaluate(wi, nl, wo);
+        float* o = out10 + 10 * i;
+        o[0] = wo.x; o[1] = wo.y; o[2] = wo.z; o[3] = w.x; o[4] = w.y; o[5] = w.z;
+        o[6] = pdf; o[7] = ev.x; o[8] = ev.y; o[9] = ev.z;
+    }
+}
+
+// DiscreteSampler over `data` (sample.cpp:103-118) evaluated at r.
+REF_API void ref_discrete(const float* data, int n, const float* r, long long nr, int* idx, float* pdf)
+{
+    std::vector<float> d(data, data + n);
+    DiscreteSampler s(d);
+    for (int i = 0; i < n; i++) pdf[i] = s.pdf(i);
+    for (long long k = 0; k < nr; k++) idx[k] = s.sample(r[k]);
+}
