@@ -1,0 +1,151 @@
+/*
+ * mcrt.h -- C ABI of the MI355X (gfx950) ray-query core and path tracer.
+ *
+ * The boundary replaces the reference's ray-query plug-in point and its
+ * scene/integrator interface:
+ *   - SceneBVH(triangles) / intersect / occluded      src/core/scenebvh.h:11-25
+ *   - Scene::add / loadModel / setCamera / finalize   src/core/scene.h:28-103
+ *   - the demo_pt render loop + rayTrace             src/examples/pt/demo_pt.cpp:512-720
+ *   - todo_addMaterial (global material library)      src/core/material.cpp:130-139
+ *
+ * Conventions: opaque handles, every call returns an mcrt_status (0 = OK)
+ * and never exits or throws; mcrt_last_error() gives a thread-local message
+ * for the last failing call on this thread.  Caller-owned buffers; device
+ * memory for scenes is owned by the library per handle.  Buffers passed to
+ * queries and renders are host pointers unless MCRT_DEVICE_PTRS is set in
+ * `flags`, in which case they are device pointers on the context's device
+ * and the call is asynchronous on `stream` (a hipStream_t, NULL = default).
+ * Host-pointer calls are synchronous.
+ *
+ * Threading: a finalized scene is immutable and may be queried and rendered
+ * from several host threads on distinct streams.  Scene mutation and
+ * mcrt_scene_finalize are single-threaded per handle.
+ */
+#ifndef MCRT_H
+#define MCRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCRT_API __attribute__((visibility("default")))
+
+typedef enum mcrt_status {
+    MCRT_OK = 0,
+    MCRT_ERR_INVALID_ARG = 1,  /* NULL handle/pointer, bad size or enum     */
+    MCRT_ERR_STATE = 2,        /* e.g. query before finalize, add after     */
+    MCRT_ERR_IO = 3,           /* file cannot be opened or written          */
+    MCRT_ERR_DEVICE = 4,       /* HIP runtime error (message has details)   */
+    MCRT_ERR_OUT_OF_MEMORY = 5 /* host or device allocation failed         */
+} mcrt_status;
+
+/* MaterialType values of src/core/material.h:7 that the path supports. */
+typedef enum mcrt_material_type {
+    MCRT_LGHT = 0, /* area light: emitted radiance = rgb, one-sided        */
+    MCRT_DIFF = 1, /* Lambertian, albedo = rgb             bsdf.cpp:51-116 */
+    MCRT_GLSY = 4  /* Phong lobe, exponent = glossiness    bsdf.cpp:204-301 */
+} mcrt_material_type;
+
+enum {
+    MCRT_DEVICE_PTRS = 1 /* buffers are device pointers, call is async    */
+};
+
+typedef struct mcrt_ctx mcrt_ctx;
+typedef struct mcrt_scene mcrt_scene;
+
+/* One ray = 8 floats {ox, oy, oz, tmin, dx, dy, dz, tmax} (32 bytes). */
+/* One hit = {prim, t, u, v}; prim = -1 and t = +inf on a miss, as HitInfo
+ * after reset() (src/core/intersection.cpp:23-31).  u weights vertex b and
+ * v vertex c (src/core/triangle.cpp:11-51). */
+typedef struct mcrt_hit {
+    int32_t prim;
+    float t, u, v;
+} mcrt_hit;
+
+typedef struct mcrt_build_opts {
+    int32_t threads; /* host threads for the SBVH build; 0 = all cores      */
+} mcrt_build_opts;
+
+typedef struct mcrt_build_stats {
+    int64_t triangles;
+    int64_t binary_nodes;   /* SplitBVH nodes (branches + leaves)            */
+    int64_t leaves;
+    int64_t refs;           /* leaf references (spatial splits duplicate)    */
+    int64_t wide_nodes;     /* 4-wide nodes after collapsing                 */
+    int64_t max_depth;      /* binary depth                                  */
+    int64_t stack_bound;    /* traversal stack entries that always suffice   */
+    uint64_t tree_digest;   /* FNV-1a of the canonical binary tree           */
+    double build_seconds;   /* host SBVH build                               */
+    double flatten_seconds; /* collapse + upload                             */
+    int64_t device_bytes;   /* scene bytes resident in HBM                   */
+} mcrt_build_stats;
+
+typedef struct mcrt_render_params {
+    int32_t spp;           /* samples per pixel in this call                */
+    int32_t max_depth;     /* path vertices n = 0..max_depth (demo_pt.cpp:520-523) */
+    int32_t rr_start;      /* Russian roulette from vertex rr_start on      */
+    float rr_prob;         /* survival probability (1 = no roulette)        */
+    uint64_t seed;         /* counter-based RNG key                         */
+    int64_t sample_offset; /* index of the first sample (progressive runs)  */
+    int32_t nee;           /* 1 = light sampling + MIS at diffuse vertices
+                              (demo_pt default), 0 = passive (BSDF only)    */
+    int32_t reserved;
+} mcrt_render_params;
+
+MCRT_API const char* mcrt_last_error(void);
+MCRT_API const char* mcrt_version(void);
+
+/* Lifecycle.  `device` is a HIP device ordinal. */
+MCRT_API int mcrt_create(int device, mcrt_ctx** out);
+MCRT_API int mcrt_destroy(mcrt_ctx* ctx);
+MCRT_API int mcrt_device_name(mcrt_ctx* ctx, char* buf, size_t cap);
+
+/* Scene building (Scene / todo_addMaterial). */
+MCRT_API int mcrt_scene_create(mcrt_ctx* ctx, mcrt_scene** out);
+MCRT_API int mcrt_scene_destroy(mcrt_scene* scene);
+/* Material(c, e, t) + todo_addMaterial: returns the material id. */
+MCRT_API int mcrt_material_add(mcrt_scene* scene, int type, const float rgb[3], float glossiness,
+                               int32_t* out_id);
+/* Bulk replacement of material colours (albedo / emission) after finalize,
+ * for optimisation loops; rgb holds 3 floats per material id. */
+MCRT_API int mcrt_material_set_rgb(mcrt_scene* scene, const float* rgb, size_t nmat);
+/* Scene::add per triangle: v9 = n x (a, b, c), vn9 = n x (na, nb, nc) or
+ * NULL for the face normal normalize(cross(a-c, b-c)) at every corner
+ * (triangle.cpp:67-74), mat = material id per triangle. */
+MCRT_API int mcrt_scene_add_triangles(mcrt_scene* scene, const float* v9, const float* vn9,
+                                      const int32_t* mat, size_t n);
+/* Scene::loadModel(path, transform, mat): `v x y z` / `f a b c` OBJ,
+ * column-major 4x4 transform (NULL = identity), area-weighted vertex
+ * normals; out_n (optional) receives the number of faces added. */
+MCRT_API int mcrt_scene_load_obj(mcrt_scene* scene, const char* path, const float* m16,
+                                 int32_t mat, size_t* out_n);
+/* Scene::setCamera(origin, target, W, H, fov_y), up = (0, 1, 0). */
+MCRT_API int mcrt_scene_set_camera(mcrt_scene* scene, const float o[3], const float target[3],
+                                   int32_t width, int32_t height, float fov_y);
+/* Scene::finalize: SBVH build on the host, 4-wide collapse, upload. */
+MCRT_API int mcrt_scene_finalize(mcrt_scene* scene, const mcrt_build_opts* opts,
+                                 mcrt_build_stats* stats);
+MCRT_API int mcrt_scene_num_triangles(const mcrt_scene* scene, size_t* out);
+
+/* Ray queries.  mcrt_intersect = SplitBVH::intersect over each ray's own
+ * [tmin, tmax] (SceneBVH::intersect uses tmin = 1e-4, tmax = +inf).
+ * mcrt_occluded writes 1 if any hit lies in (tmin, tmax)
+ * (SceneBVH::occluded = the same with tmax = tfar - 1e-4). */
+MCRT_API int mcrt_intersect(mcrt_scene* scene, const float* rays, size_t n, mcrt_hit* hits,
+                            int flags, void* stream);
+MCRT_API int mcrt_occluded(mcrt_scene* scene, const float* rays, size_t n, uint8_t* occluded,
+                           int flags, void* stream);
+
+/* Wavefront path tracer.  film = W*H*3 floats, row y = 0 is the image
+ * bottom (film.cpp:204-215); each pixel receives the mean of its samples'
+ * clamped radiance (demo_pt.cpp:700-710). */
+MCRT_API int mcrt_render(mcrt_scene* scene, const mcrt_render_params* params, float* film,
+                         int flags, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCRT_H */

@@ -1,0 +1,237 @@
+"""ctypes binding of include/mcrt.h and a Scene facade that mirrors the
+reference's Scene (src/core/scene.h:28-103): add / loadModel / setCamera /
+finalize / intersect / occluded, plus render().
+
+The binding loads mcrt_amd/libmcrt_hip.so and raises if it is missing; there
+is no CPU fallback.  Buffers are numpy arrays (host) or torch tensors on the
+GPU (device pointers, asynchronous on the current stream).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmcrt_hip.so")
+
+LGHT, DIFF, GLSY = 0, 1, 4
+DEVICE_PTRS = 1
+
+ERRORS = {1: "invalid argument", 2: "invalid state", 3: "I/O error", 4: "device error", 5: "out of memory"}
+
+
+class McrtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mcrt error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class BuildStats(ctypes.Structure):
+    _fields_ = [("triangles", ctypes.c_int64), ("binary_nodes", ctypes.c_int64), ("leaves", ctypes.c_int64),
+                ("refs", ctypes.c_int64), ("wide_nodes", ctypes.c_int64), ("max_depth", ctypes.c_int64),
+                ("stack_bound", ctypes.c_int64), ("tree_digest", ctypes.c_uint64),
+                ("build_seconds", ctypes.c_double), ("flatten_seconds", ctypes.c_double),
+                ("device_bytes", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class BuildOpts(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_int32)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("spp", ctypes.c_int32), ("max_depth", ctypes.c_int32), ("rr_start", ctypes.c_int32),
+                ("rr_prob", ctypes.c_float), ("seed", ctypes.c_uint64), ("sample_offset", ctypes.c_int64),
+                ("nee", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+_lib_handle = None
+
+
+def lib():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib_handle
+    if _lib_handle is not None:
+        return _lib_handle
+    if not os.path.exists(LIB_PATH):
+        raise McrtError(4, f"{LIB_PATH} is missing: run `python -m mcrt_amd.build` first")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, Z = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+    sig = {
+        "mcrt_last_error": (ctypes.c_char_p, []),
+        "mcrt_version": (ctypes.c_char_p, []),
+        "mcrt_create": (I, [I, ctypes.POINTER(P)]),
+        "mcrt_destroy": (I, [P]),
+        "mcrt_device_name": (I, [P, ctypes.c_char_p, Z]),
+        "mcrt_scene_create": (I, [P, ctypes.POINTER(P)]),
+        "mcrt_scene_destroy": (I, [P]),
+        "mcrt_material_add": (I, [P, I, P, ctypes.c_float, ctypes.POINTER(ctypes.c_int32)]),
+        "mcrt_material_set_rgb": (I, [P, P, Z]),
+        "mcrt_scene_add_triangles": (I, [P, P, P, P, Z]),
+        "mcrt_scene_load_obj": (I, [P, ctypes.c_char_p, P, ctypes.c_int32, ctypes.POINTER(Z)]),
+        "mcrt_scene_set_camera": (I, [P, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float]),
+        "mcrt_scene_finalize": (I, [P, ctypes.POINTER(BuildOpts), ctypes.POINTER(BuildStats)]),
+        "mcrt_scene_num_triangles": (I, [P, ctypes.POINTER(Z)]),
+        "mcrt_intersect": (I, [P, P, Z, P, I, P]),
+        "mcrt_occluded": (I, [P, P, Z, P, I, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib_handle = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise McrtError(rc, lib().mcrt_last_error().decode())
+
+
+def _ptr(a):
+    """(pointer, is_device) of a numpy array or torch tensor."""
+    if a is None:
+        return None, False
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data, False
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return a.data_ptr(), a.is_cuda
+    raise TypeError(f"unsupported buffer type {type(a)}")
+
+
+def _stream_of(dev):
+    if not dev:
+        return None
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Context:
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().mcrt_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def name(self):
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().mcrt_device_name(self.h, buf, 256))
+        return buf.value.decode()
+
+    def close(self):
+        if self.h:
+            _check(lib().mcrt_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Mirror of Scene (scene.h): build, finalize, query, render."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().mcrt_scene_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+        self.stats = None
+        self.camera = None
+
+    # -- building -------------------------------------------------------
+    def add_material(self, type_, rgb, glossiness=25.0):
+        """Material(c, e, t) + todo_addMaterial (material.cpp:5-27, 130-139)."""
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(3)
+        out = ctypes.c_int32()
+        _check(lib().mcrt_material_add(self.h, int(type_), rgb.ctypes.data, float(glossiness), ctypes.byref(out)))
+        return out.value
+
+    def set_material_rgb(self, rgb):
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+        _check(lib().mcrt_material_set_rgb(self.h, rgb.ctypes.data, rgb.shape[0]))
+
+    def add(self, tris, mat, normals=None):
+        """Scene::add for n triangles: tris (n,3,3), mat int or (n,)."""
+        tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        n = tris.shape[0]
+        mats = np.ascontiguousarray(np.broadcast_to(np.asarray(mat, np.int32), (n,)))
+        vn = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
+        _check(lib().mcrt_scene_add_triangles(self.h, tris.ctypes.data, None if vn is None else vn.ctypes.data,
+                                              mats.ctypes.data, n))
+        return n
+
+    def load_model(self, path, mat, transform=None):
+        """Scene::loadModel(path, transform (column-major 4x4), mat)."""
+        m = None if transform is None else np.ascontiguousarray(np.asarray(transform, np.float32).T.reshape(16))
+        n = ctypes.c_size_t()
+        _check(lib().mcrt_scene_load_obj(self.h, os.fsencode(path), None if m is None else m.ctypes.data, int(mat),
+                                         ctypes.byref(n)))
+        return n.value
+
+    def set_camera(self, origin, target, width, height, fov_y):
+        o = np.ascontiguousarray(origin, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        _check(lib().mcrt_scene_set_camera(self.h, o.ctypes.data, t.ctypes.data, int(width), int(height),
+                                           float(fov_y)))
+        self.camera = (tuple(o.tolist()), tuple(t.tolist()), int(width), int(height), float(fov_y))
+
+    def finalize(self, threads=0):
+        st = BuildStats()
+        opts = BuildOpts(threads)
+        _check(lib().mcrt_scene_finalize(self.h, ctypes.byref(opts), ctypes.byref(st)))
+        self.stats = st.as_dict()
+        return self.stats
+
+    def num_triangles(self):
+        n = ctypes.c_size_t()
+        _check(lib().mcrt_scene_num_triangles(self.h, ctypes.byref(n)))
+        return n.value
+
+    # -- queries ----------------------------------------------------------
+    def intersect(self, rays, hits=None):
+        """Closest hit for rays (n,8) -> hits (n,4) int32 view {prim, t, u, v}."""
+        n = rays.shape[0]
+        if hits is None:
+            hits = np.empty((n, 4), np.int32)
+        rp, rdev = _ptr(rays)
+        hp, hdev = _ptr(hits)
+        if rdev != hdev:
+            raise ValueError("rays and hits must both be host or both be device buffers")
+        _check(lib().mcrt_intersect(self.h, rp, n, hp, DEVICE_PTRS if rdev else 0, _stream_of(rdev)))
+        return hits
+
+    def occluded(self, rays, out=None):
+        n = rays.shape[0]
+        if out is None:
+            out = np.empty(n, np.uint8)
+        rp, rdev = _ptr(rays)
+        op, odev = _ptr(out)
+        if rdev != odev:
+            raise ValueError("rays and output must both be host or both be device buffers")
+        _check(lib().mcrt_occluded(self.h, rp, n, op, DEVICE_PTRS if rdev else 0, _stream_of(rdev)))
+        return out
+
+    def close(self):
+        if self.h:
+            _check(lib().mcrt_scene_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def split_hits(hits):
+    """(prim int32, tuv float32 (n,3)) views of an (n,4) int32 hit array."""
+    h = np.asarray(hits)
+    return h[:, 0].copy(), h[:, 1:].copy().view(np.float32)

@@ -1,0 +1,278 @@
+// C-ABI implementation (include/mcrt.h): scene ingestion, host SBVH build,
+// 4-wide collapse, HBM upload and the query / render entry points.
+#include "mcrt.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "capi_internal.h"
+
+using namespace mcrt;
+
+namespace
+{
+thread_local std::string g_err;
+}
+
+int mcrt::fail(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int mcrt::ok()
+{
+    return MCRT_OK;
+}
+
+mcrt::DevBuf::~DevBuf() { release(); }
+
+void mcrt::DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int mcrt::DevBuf::ensure(size_t want)
+{
+    if (want <= bytes && p) return MCRT_OK;
+    release();
+    if (want == 0) return MCRT_OK;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess)
+    {
+        p = nullptr;
+        (void)hipGetLastError();
+        return fail(MCRT_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    bytes = want;
+    return MCRT_OK;
+}
+
+int mcrt::DevBuf::upload(const void* src, size_t n)
+{
+    int rc = ensure(n);
+    if (rc) return rc;
+    if (n == 0) return MCRT_OK;
+    hipError_t e = hipMemcpy(p, src, n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(MCRT_ERR_DEVICE, "hipMemcpy H2D: %s", hipGetErrorString(e));
+    return MCRT_OK;
+}
+
+extern "C" {
+
+MCRT_API const char* mcrt_last_error(void) { return g_err.c_str(); }
+MCRT_API const char* mcrt_version(void) { return "mcrt-mi355x 0.1 (gfx950)"; }
+
+MCRT_API int mcrt_create(int device, mcrt_ctx** out)
+{
+    if (!out) return fail(MCRT_ERR_INVALID_ARG, "mcrt_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+    {
+        (void)hipGetLastError();
+        return fail(MCRT_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    }
+    if (device < 0 || device >= n) return fail(MCRT_ERR_INVALID_ARG, "device %d out of range [0,%d)", device, n);
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    auto* c = new (std::nothrow) mcrt_ctx();
+    if (!c) return fail(MCRT_ERR_OUT_OF_MEMORY, "mcrt_create: out of host memory");
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    c->name = prop.gcnArchName;
+    *out = c;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_destroy(mcrt_ctx* ctx)
+{
+    if (!ctx) return fail(MCRT_ERR_INVALID_ARG, "mcrt_destroy: NULL context");
+    delete ctx;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_device_name(mcrt_ctx* ctx, char* buf, size_t cap)
+{
+    if (!ctx || !buf || cap == 0) return fail(MCRT_ERR_INVALID_ARG, "mcrt_device_name: bad arguments");
+    snprintf(buf, cap, "%s (%d CUs)", ctx->name.c_str(), ctx->num_cus);
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_scene_create(mcrt_ctx* ctx, mcrt_scene** out)
+{
+    if (!ctx || !out) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_create: NULL argument");
+    auto* s = new (std::nothrow) mcrt_scene();
+    if (!s) return fail(MCRT_ERR_OUT_OF_MEMORY, "mcrt_scene_create: out of host memory");
+    s->ctx = ctx;
+    *out = s;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_scene_destroy(mcrt_scene* s)
+{
+    if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_destroy: NULL scene");
+    (void)hipSetDevice(s->ctx->device);
+    delete s;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_material_add(mcrt_scene* s, int type, const float rgb[3], float glossiness, int32_t* out_id)
+{
+    if (!s || !rgb) return fail(MCRT_ERR_INVALID_ARG, "mcrt_material_add: NULL argument");
+    if (s->finalized) return fail(MCRT_ERR_STATE, "mcrt_material_add: scene already finalized");
+    if (type != MCRT_LGHT && type != MCRT_DIFF && type != MCRT_GLSY)
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_material_add: unsupported material type %d", type);
+    if (type == MCRT_GLSY && !(glossiness >= 0.0f))
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_material_add: glossiness must be >= 0");
+    HostMaterial m;
+    m.type = type;
+    m.rgb[0] = rgb[0]; m.rgb[1] = rgb[1]; m.rgb[2] = rgb[2];
+    m.gloss = glossiness;
+    s->mats.push_back(m);
+    if (out_id) *out_id = (int32_t)s->mats.size() - 1;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_material_set_rgb(mcrt_scene* s, const float* rgb, size_t nmat)
+{
+    if (!s || (!rgb && nmat)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_material_set_rgb: NULL argument");
+    if (nmat != s->mats.size())
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_material_set_rgb: %zu colours for %zu materials", nmat,
+                    s->mats.size());
+    for (size_t i = 0; i < nmat; i++)
+        for (int k = 0; k < 3; k++) s->mats[i].rgb[k] = rgb[3 * i + k];
+    if (s->finalized)
+    {
+        HIPCHK(hipSetDevice(s->ctx->device));
+        int rc = upload_materials(s);
+        if (rc) return rc;
+    }
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_scene_add_triangles(mcrt_scene* s, const float* v9, const float* vn9, const int32_t* mat,
+                                      size_t n)
+{
+    if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_add_triangles: NULL scene");
+    if (n && (!v9 || !mat)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_add_triangles: NULL array");
+    if (s->finalized) return fail(MCRT_ERR_STATE, "mcrt_scene_add_triangles: scene already finalized");
+    for (size_t i = 0; i < n; i++)
+        if (mat[i] < 0 || (size_t)mat[i] >= s->mats.size())
+            return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_add_triangles: material id %d of triangle %zu is unknown",
+                        mat[i], i);
+    for (size_t i = 0; i < n; i++)
+    {
+        Tri t;
+        for (int k = 0; k < 3; k++) { t.a[k] = v9[9 * i + k]; t.b[k] = v9[9 * i + 3 + k]; t.c[k] = v9[9 * i + 6 + k]; }
+        s->tris.push_back(t);
+        if (vn9)
+            for (int k = 0; k < 9; k++) s->vn.push_back(vn9[9 * i + k]);
+        else
+        {
+            F3 fn = normalize3(cross3(sub(t.a, t.c), sub(t.b, t.c)));  // triangle.cpp:67-74
+            for (int c = 0; c < 3; c++)
+                for (int k = 0; k < 3; k++) s->vn.push_back(fn[k]);
+        }
+        s->tri_mat.push_back(mat[i]);
+    }
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_scene_load_obj(mcrt_scene* s, const char* path, const float* m16, int32_t mat, size_t* out_n)
+{
+    if (!s || !path) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_load_obj: NULL argument");
+    if (s->finalized) return fail(MCRT_ERR_STATE, "mcrt_scene_load_obj: scene already finalized");
+    if (mat < 0 || (size_t)mat >= s->mats.size())
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_load_obj: material id %d is unknown", mat);
+    ObjMesh m;
+    int rc = load_obj(path, m16, m);
+    if (rc) return rc;
+    const size_t nf = m.f.size() / 3;
+    std::vector<float> v9(9 * nf), vn9(9 * nf);
+    for (size_t i = 0; i < nf; i++)
+        for (int c = 0; c < 3; c++)
+        {
+            int vi = m.f[3 * i + c];
+            for (int k = 0; k < 3; k++)
+            {
+                v9[9 * i + 3 * c + k] = m.v[vi][k];
+                vn9[9 * i + 3 * c + k] = m.vn[vi][k];
+            }
+        }
+    std::vector<int32_t> mats(nf, mat);
+    rc = mcrt_scene_add_triangles(s, v9.data(), vn9.data(), mats.data(), nf);
+    if (rc) return rc;
+    if (out_n) *out_n = nf;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_scene_set_camera(mcrt_scene* s, const float o[3], const float target[3], int32_t width,
+                                   int32_t height, float fov_y)
+{
+    if (!s || !o || !target) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_set_camera: NULL argument");
+    if (width <= 0 || height <= 0) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_set_camera: bad film size");
+    if (!(fov_y > 0.0f && fov_y < 180.0f)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_set_camera: bad fov");
+    s->has_cam = true;
+    for (int k = 0; k < 3; k++) { s->cam_o[k] = o[k]; s->cam_t[k] = target[k]; }
+    s->cam_w = width;
+    s->cam_h = height;
+    s->cam_fov = fov_y;
+    s->dev_cam = make_camera(s);
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_scene_num_triangles(const mcrt_scene* s, size_t* out)
+{
+    if (!s || !out) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_num_triangles: NULL argument");
+    *out = s->tris.size();
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_scene_finalize(mcrt_scene* s, const mcrt_build_opts* opts, mcrt_build_stats* stats)
+{
+    if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_finalize: NULL scene");
+    if (s->finalized) return fail(MCRT_ERR_STATE, "mcrt_scene_finalize: scene already finalized");
+    HIPCHK(hipSetDevice(s->ctx->device));
+    int rc = finalize_scene(s, opts ? opts->threads : 0);
+    if (rc) return rc;
+    s->finalized = true;
+    if (stats) *stats = s->stats;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_intersect(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, int flags, void* stream)
+{
+    if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_intersect: NULL scene");
+    if (n && (!rays || !hits)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_intersect: NULL buffer");
+    return run_query(s, rays, n, hits, nullptr, TRACE_CLOSEST, flags, (hipStream_t)stream, nullptr);
+}
+
+MCRT_API int mcrt_occluded(mcrt_scene* s, const float* rays, size_t n, uint8_t* occluded, int flags, void* stream)
+{
+    if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_occluded: NULL scene");
+    if (n && (!rays || !occluded)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_occluded: NULL buffer");
+    return run_query(s, rays, n, nullptr, occluded, TRACE_ANY, flags, (hipStream_t)stream, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// Internal types behind the opaque handles of include/mcrt.h.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "geom.h"
+#include "mcrt.h"
+
+struct mcrt_ctx
+{
+    int device = 0;
+    int num_cus = 0;
+    std::string name;
+};
+
+namespace mcrt
+{
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int ok();
+
+#define HIPCHK(x)                                                                              \
+    do                                                                                         \
+    {                                                                                          \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+        {                                                                                      \
+            (void)hipGetLastError();                                                           \
+            return ::mcrt::fail(MCRT_ERR_DEVICE, "%s failed: %s", #x, hipGetErrorString(e_)); \
+        }                                                                                      \
+    } while (0)
+
+// Owned device allocation.
+struct DevBuf
+{
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf();
+    void release();
+    int ensure(size_t want);
+    int upload(const void* src, size_t n);
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct HostMaterial
+{
+    int type;
+    float rgb[3];
+    float gloss;
+};
+
+struct ObjMesh
+{
+    std::vector<F3> v, vn;
+    std::vector<int> f;  // 3 per face, 0-based
+};
+int load_obj(const char* path, const float* m16, ObjMesh& out);
+}  // namespace mcrt
+
+struct mcrt_scene
+{
+    mcrt_ctx* ctx = nullptr;
+    // host description
+    std::vector<mcrt::Tri> tris;
+    std::vector<float> vn;  // 9 per triangle
+    std::vector<int32_t> tri_mat;
+    std::vector<mcrt::HostMaterial> mats;
+    bool has_cam = false;
+    float cam_o[3] = {0, 0, 0}, cam_t[3] = {0, 0, 1};
+    int32_t cam_w = 0, cam_h = 0;
+    float cam_fov = 45.0f;
+    mcrt::DevCamera dev_cam;
+    bool finalized = false;
+    mcrt_build_stats stats{};
+    // device residency
+    mcrt::DevScene dev;
+    mcrt::DevBuf d_nodes, d_refs, d_shade, d_verts, d_mats, d_lights, d_spill;
+    // staging for host-pointer queries
+    std::mutex q_mu;
+    mcrt::DevBuf q_in, q_out;
+};
+
+namespace mcrt
+{
+DevCamera make_camera(const mcrt_scene* s);
+int finalize_scene(mcrt_scene* s, int threads);
+int upload_materials(mcrt_scene* s);
+int run_query(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, uint8_t* occ, int mode, int flags,
+              hipStream_t stream, uint64_t* counters);
+}  // namespace mcrt

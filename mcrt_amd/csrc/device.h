@@ -1,0 +1,67 @@
+// Device-side scene handle and launch helpers shared by the HIP sources
+// and the host C-ABI layer.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "layout.h"
+
+namespace mcrt
+{
+struct DevMaterial
+{
+    float rgb[3];
+    int32_t type;  // mcrt_material_type
+    float gloss;
+    float pad[3];
+};
+
+struct DevLight
+{
+    int32_t tri;
+    float area;  // TriangleObject::area
+    float pdf;   // DiscreteSampler pdf
+    float cdf;   // DiscreteSampler cdf
+};
+
+struct DevCamera
+{
+    float o[3], fwd[3], fx[3], fy[3];
+    float dist, ww;
+    int32_t w, h;
+};
+
+struct DevScene
+{
+    const WNode* nodes = nullptr;
+    const WRef* refs = nullptr;
+    const TriShade* shade = nullptr;   // per triangle
+    const float* verts = nullptr;      // per triangle a, b, c (9 floats)
+    const DevMaterial* mats = nullptr;
+    const DevLight* lights = nullptr;
+    int32_t nlights = 0;
+    int32_t stack_bound = 0;
+    int64_t ntri = 0;
+    uint64_t* spill = nullptr;         // traversal stack overflow storage
+    int64_t spill_lanes = 0;           // lanes the spill buffer covers
+    int32_t spill_depth = 0;           // entries per lane beyond the LDS stack
+    int32_t lds_depth = 16;            // LDS stack entries per lane
+};
+
+// Traversal modes.
+enum TraceMode
+{
+    TRACE_CLOSEST = 0,
+    TRACE_ANY = 1
+};
+
+// Launches a query over n rays (8 floats each, device memory).  Closest:
+// hits = int4 {prim, t, u, v} per ray.  Any: occ = 1 byte per ray.
+// counters (optional, device, 4 x uint64): node fetches, ref fetches,
+// MT tests, rays.
+hipError_t launch_trace(const DevScene& s, const float* rays, int64_t n, void* hits, uint8_t* occ,
+                        int mode, uint64_t* counters, hipStream_t stream);
+
+// Lanes a trace launch keeps resident (the spill buffer must cover them).
+int64_t trace_lanes(int num_cus);
+}  // namespace mcrt

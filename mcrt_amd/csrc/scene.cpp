@@ -1,0 +1,232 @@
+// Scene ingestion (OBJ), finalize (SBVH build + collapse + upload), light
+// tables, camera set-up and the query dispatcher.
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cmath>
+#include <fstream>
+#include <sstream>
+
+#include "capi_internal.h"
+#include "sbvh.h"
+#include "wide.h"
+
+namespace mcrt
+{
+// Mesh::Mesh (mesh.cpp:24-60): whitespace tokens; `v x y z`, `f a b c`
+// (1-based), everything else skipped token by token.  The optional
+// column-major transform is applied per vertex in GLM's order
+// (m0*x + m1*y) + (m2*z + m3*w) (type_mat4x4.inl:532-543), then
+// Mesh::genNormals (mesh.cpp:66-92): vertex normal = normalize of the sum
+// of the unnormalised face normals (vb-va) x (vc-va) in face order.
+int load_obj(const char* path, const float* m16, ObjMesh& m)
+{
+    std::ifstream ifs(path, std::ios::binary);
+    if (!ifs.good()) return fail(MCRT_ERR_IO, "cannot open %s", path);
+    std::stringstream reader;
+    reader << ifs.rdbuf();
+    std::string token;
+    while (!reader.eof())
+    {
+        reader >> token;
+        if (token == "v")
+        {
+            float a, b, c;
+            reader >> a >> b >> c;
+            m.v.push_back(f3(a, b, c));
+        }
+        else if (token == "f")
+        {
+            unsigned int a, b, c;
+            reader >> a >> b >> c;
+            m.f.push_back((int)(a - 1));
+            m.f.push_back((int)(b - 1));
+            m.f.push_back((int)(c - 1));
+        }
+    }
+    for (int idx : m.f)
+        if (idx < 0 || (size_t)idx >= m.v.size())
+            return fail(MCRT_ERR_IO, "%s: face index %d out of range (%zu vertices)", path, idx + 1, m.v.size());
+    if (m16)
+        for (F3& p : m.v)
+        {
+            const float x = p[0], y = p[1], z = p[2], w = 1.0f;
+            F3 q;
+            for (int r = 0; r < 3; r++)
+                q[r] = (m16[0 * 4 + r] * x + m16[1 * 4 + r] * y) + (m16[2 * 4 + r] * z + m16[3 * 4 + r] * w);
+            p = q;
+        }
+    m.vn.assign(m.v.size(), f3(0, 0, 0));
+    for (size_t i = 0; i < m.f.size() / 3; i++)
+    {
+        int a = m.f[3 * i], b = m.f[3 * i + 1], c = m.f[3 * i + 2];
+        F3 fn = cross3(sub(m.v[b], m.v[a]), sub(m.v[c], m.v[a]));
+        m.vn[a] = add(m.vn[a], fn);
+        m.vn[b] = add(m.vn[b], fn);
+        m.vn[c] = add(m.vn[c], fn);
+    }
+    for (F3& n : m.vn) n = normalize3(n);
+    return MCRT_OK;
+}
+
+// Camera::Camera (camera.cpp:9-35), up = (0,1,0) (scene.cpp:231).
+DevCamera make_camera(const mcrt_scene* s)
+{
+    DevCamera c;
+    F3 o = f3(s->cam_o[0], s->cam_o[1], s->cam_o[2]);
+    F3 at = f3(s->cam_t[0], s->cam_t[1], s->cam_t[2]);
+    c.ww = float(s->cam_w) / float(s->cam_h);
+    float rad = (s->cam_fov * 0.5f) / 180.0f * 3.14159265358979323846f;  // toRadians (numeric.cpp:47-50)
+    c.dist = 1.0f / (2.0f * std::tan(rad));
+    F3 fwd = normalize3(sub(at, o));
+    F3 fx = normalize3(cross3(fwd, f3(0, 1, 0)));
+    F3 fy = cross3(fx, fwd);
+    for (int k = 0; k < 3; k++) { c.o[k] = o[k]; c.fwd[k] = fwd[k]; c.fx[k] = fx[k]; c.fy[k] = fy[k]; }
+    c.w = s->cam_w;
+    c.h = s->cam_h;
+    return c;
+}
+
+int upload_materials(mcrt_scene* s)
+{
+    std::vector<DevMaterial> dm(s->mats.size());
+    for (size_t i = 0; i < s->mats.size(); i++)
+    {
+        for (int k = 0; k < 3; k++) dm[i].rgb[k] = s->mats[i].rgb[k];
+        dm[i].type = s->mats[i].type;
+        dm[i].gloss = s->mats[i].gloss;
+        dm[i].pad[0] = dm[i].pad[1] = dm[i].pad[2] = 0.0f;
+    }
+    int rc = s->d_mats.upload(dm.data(), dm.size() * sizeof(DevMaterial));
+    if (rc) return rc;
+    s->dev.mats = s->d_mats.as<DevMaterial>();
+    return MCRT_OK;
+}
+
+int finalize_scene(mcrt_scene* s, int threads)
+{
+    using clk = std::chrono::steady_clock;
+    const size_t n = s->tris.size();
+    mcrt_build_stats st{};
+    st.triangles = (int64_t)n;
+    auto t0 = clk::now();
+    BTree bt;
+    build_sbvh(s->tris, threads, bt);
+    auto t1 = clk::now();
+    st.binary_nodes = (int64_t)bt.nodes.size();
+    for (const BNode& b : bt.nodes) st.leaves += b.leaf ? 1 : 0;
+    st.refs = (int64_t)bt.refs.size();
+    st.max_depth = bt.max_depth;
+    st.tree_digest = tree_digest(bt);
+    WideBVH wb;
+    collapse_wide(bt, s->tris, wb);
+    { BTree().nodes.swap(bt.nodes); std::vector<SRef>().swap(bt.refs); }
+    st.wide_nodes = (int64_t)wb.nodes.size();
+    st.stack_bound = wb.stack_bound;
+
+    // Per-triangle shading records and the light list (Scene::finalize,
+    // scene.cpp:151-178): lights are LGHT triangles in index order, picked
+    // proportionally to area with DiscreteSampler's fp32 normalisation.
+    std::vector<TriShade> shade(n);
+    std::vector<float> verts(9 * n);
+    std::vector<DevLight> lights;
+    for (size_t i = 0; i < n; i++)
+    {
+        TriShade& t = shade[i];
+        for (int k = 0; k < 3; k++)
+        {
+            t.na[k] = s->vn[9 * i + k];
+            t.nb[k] = s->vn[9 * i + 3 + k];
+            t.nc[k] = s->vn[9 * i + 6 + k];
+            verts[9 * i + k] = s->tris[i].a[k];
+            verts[9 * i + 3 + k] = s->tris[i].b[k];
+            verts[9 * i + 6 + k] = s->tris[i].c[k];
+        }
+        t.mat = s->tri_mat[i];
+        t.light = -1;
+        t.pad = 0.0f;
+        if (s->mats[t.mat].type == MCRT_LGHT)
+        {
+            t.light = (int32_t)lights.size();
+            DevLight L;
+            L.tri = (int32_t)i;
+            const Tri& tr = s->tris[i];
+            F3 cr = cross3(sub(tr.b, tr.a), sub(tr.c, tr.a));  // TriangleObject::area (triangle.cpp:129-132)
+            L.area = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) * 0.5f;
+            lights.push_back(L);
+        }
+    }
+    if (!lights.empty())  // DiscreteSampler::update (sample.cpp:103-118)
+    {
+        float norm = 0.0f;
+        for (const DevLight& L : lights) norm = norm + L.area;
+        const float k = 1.0f / norm;
+        float sum = 0.0f;
+        for (DevLight& L : lights)
+        {
+            L.pdf = L.area * k;
+            sum += L.pdf;
+            L.cdf = sum;
+        }
+    }
+
+    int rc;
+    if ((rc = s->d_nodes.upload(wb.nodes.data(), wb.nodes.size() * sizeof(WNode)))) return rc;
+    if ((rc = s->d_refs.upload(wb.refs.data(), wb.refs.size() * sizeof(WRef)))) return rc;
+    if ((rc = s->d_shade.upload(shade.data(), shade.size() * sizeof(TriShade)))) return rc;
+    if ((rc = s->d_verts.upload(verts.data(), verts.size() * sizeof(float)))) return rc;
+    if ((rc = s->d_lights.upload(lights.data(), lights.size() * sizeof(DevLight)))) return rc;
+    if ((rc = upload_materials(s))) return rc;
+
+    DevScene& d = s->dev;
+    d.nodes = wb.nodes.empty() ? nullptr : s->d_nodes.as<WNode>();
+    d.refs = s->d_refs.as<WRef>();
+    d.shade = s->d_shade.as<TriShade>();
+    d.verts = s->d_verts.as<float>();
+    d.lights = s->d_lights.as<DevLight>();
+    d.nlights = (int32_t)lights.size();
+    d.stack_bound = wb.stack_bound;
+    d.ntri = (int64_t)n;
+    d.lds_depth = 16;
+    d.spill_lanes = trace_lanes(s->ctx->num_cus);
+    d.spill_depth = wb.stack_bound > d.lds_depth ? wb.stack_bound - d.lds_depth : 0;
+    if (d.spill_depth > 0)
+    {
+        if ((rc = s->d_spill.ensure((size_t)d.spill_lanes * d.spill_depth * sizeof(uint64_t)))) return rc;
+        d.spill = s->d_spill.as<uint64_t>();
+    }
+    auto t2 = clk::now();
+    st.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    st.flatten_seconds = std::chrono::duration<double>(t2 - t1).count();
+    st.device_bytes = (int64_t)(s->d_nodes.bytes + s->d_refs.bytes + s->d_shade.bytes + s->d_verts.bytes +
+                                s->d_lights.bytes + s->d_mats.bytes + s->d_spill.bytes);
+    s->stats = st;
+    return MCRT_OK;
+}
+
+int run_query(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, uint8_t* occ, int mode, int flags,
+              hipStream_t stream, uint64_t* counters)
+{
+    if (!s->finalized) return fail(MCRT_ERR_STATE, "ray query before mcrt_scene_finalize");
+    if (n == 0) return MCRT_OK;
+    HIPCHK(hipSetDevice(s->ctx->device));
+    if (flags & MCRT_DEVICE_PTRS)
+    {
+        HIPCHK(launch_trace(s->dev, rays, (int64_t)n, hits, occ, mode, counters, stream));
+        return MCRT_OK;
+    }
+    std::lock_guard<std::mutex> g(s->q_mu);
+    const size_t in_bytes = n * 8 * sizeof(float);
+    const size_t out_bytes = mode == TRACE_ANY ? n : n * sizeof(mcrt_hit);
+    int rc;
+    if ((rc = s->q_in.ensure(in_bytes))) return rc;
+    if ((rc = s->q_out.ensure(out_bytes))) return rc;
+    HIPCHK(hipMemcpyAsync(s->q_in.p, rays, in_bytes, hipMemcpyHostToDevice, stream));
+    HIPCHK(launch_trace(s->dev, s->q_in.as<float>(), (int64_t)n, mode == TRACE_ANY ? nullptr : s->q_out.p,
+                        mode == TRACE_ANY ? s->q_out.as<uint8_t>() : nullptr, mode, counters, stream));
+    HIPCHK(hipMemcpyAsync(mode == TRACE_ANY ? (void*)occ : (void*)hits, s->q_out.p, out_bytes,
+                          hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return MCRT_OK;
+}
+}  // namespace mcrt

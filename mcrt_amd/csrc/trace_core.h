@@ -1,0 +1,267 @@
+// Device traversal core: reference-order closest-hit / any-hit over the
+// 4-wide SBVH (layout.h).  Included by every kernel that casts rays.
+//
+// Semantics are those of SplitBVH::intersect (splitbvh.cpp:5-145,
+// :1627-1642): depth-first, near child first iff dir[split_axis] > 0, a
+// node's box tested at entry against the *current* tmax, leaf references
+// tested in slot order against their clipped boxes and then by
+// Moller-Trumbore, a hit accepted iff tmin < t < tmax, tmax shrinking to
+// every accepted t (so among equal t the first visited wins).  The 4-wide
+// node tests its children when it is visited and a child's entry distance
+// `lo` travels on the stack: the reference's box test depends on tmax only
+// through `lo <= tmax`, so re-checking `lo` when the entry is popped is the
+// reference's entry-time test exactly.  Any-hit stops at the first accepted
+// hit, which is the first hit of the same traversal, so its boolean equals
+// SceneBVH::occluded's (scenebvh.cpp:379-396).
+//
+// Arithmetic is the reference's: slab t = (bound - o) * inv_dir with
+// inv_dir = (d == 0 ? 0 : 1/d) (ray.cpp:77-84), MT with the absolute
+// 1e-6 determinant cut (triangle.cpp:13-51); all fp32 with no contraction
+// and correctly rounded division.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+// Host+device so the identical traversal can also run in a host test build.
+#define MCRT_HD __host__ __device__ __forceinline__
+
+namespace mcrt
+{
+MCRT_HD uint32_t bits_of(float f) { return __builtin_bit_cast(uint32_t, f); }
+MCRT_HD float float_of(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+struct RayPrep
+{
+    float ox, oy, oz, dx, dy, dz;
+    float ix, iy, iz;
+    float R0;
+    bool sx, sy, sz;
+    bool exact;  // an inv_dir component is infinite: use the reference's
+                 // sequential test with its NaN behaviour
+    uint32_t oct;
+};
+
+MCRT_HD float inv_or_zero(float d) { return d == 0.0f ? 0.0f : 1.0f / d; }
+
+MCRT_HD RayPrep prep_ray(float ox, float oy, float oz, float dx, float dy, float dz,
+                                            float tmin)
+{
+    RayPrep r;
+    r.ox = ox; r.oy = oy; r.oz = oz;
+    r.dx = dx; r.dy = dy; r.dz = dz;
+    r.ix = inv_or_zero(dx); r.iy = inv_or_zero(dy); r.iz = inv_or_zero(dz);
+    r.sx = dx < 0.0f; r.sy = dy < 0.0f; r.sz = dz < 0.0f;
+    r.R0 = tmin;
+    r.exact = __builtin_isinf(r.ix) || __builtin_isinf(r.iy) || __builtin_isinf(r.iz);
+    r.oct = (dx > 0.0f ? 1u : 0u) | (dy > 0.0f ? 2u : 0u) | (dz > 0.0f ? 4u : 0u);
+    return r;
+}
+
+// rayAABBOverlap (aabb.cpp:128-191).  Returns pass and the entry distance
+// `lo` (the final tmin of the reference's sequence).  With finite inv_dir no
+// NaN can arise and the sequence reduces to lo <= hi && lo <= R1 && hi >= R0.
+MCRT_HD bool slab(const RayPrep& r, float lx, float hx, float ly, float hy, float lz,
+                                     float hz, float R1, float& lo_out)
+{
+    float ax = (lx - r.ox) * r.ix, bx = (hx - r.ox) * r.ix;
+    float ay = (ly - r.oy) * r.iy, by = (hy - r.oy) * r.iy;
+    float az = (lz - r.oz) * r.iz, bz = (hz - r.oz) * r.iz;
+    float t0x = r.sx ? bx : ax, t1x = r.sx ? ax : bx;
+    float t0y = r.sy ? by : ay, t1y = r.sy ? ay : by;
+    float t0z = r.sz ? bz : az, t1z = r.sz ? az : bz;
+    if (!r.exact)
+    {
+        float lo = fmaxf(fmaxf(t0x, t0y), t0z);
+        float hi = fminf(fminf(t1x, t1y), t1z);
+        lo_out = lo;
+        return lo <= hi && lo <= R1 && hi >= r.R0;
+    }
+    if ((t0x > t1y) || (t0y > t1x)) return false;
+    float tmin = t0y > t0x ? t0y : t0x;
+    if (tmin > R1) return false;
+    float tmax = t1y < t1x ? t1y : t1x;
+    if (tmax < r.R0) return false;
+    if ((tmin > t1z) || (t0z > tmax)) return false;
+    if (t0z > tmin) tmin = t0z;
+    if (tmin > R1) return false;
+    if (t1z < tmax) tmax = t1z;
+    if (tmax < r.R0) return false;
+    lo_out = tmin;
+    return true;
+}
+
+// rayTriangleIntersect (triangle.cpp:13-51), e1/e2 precomputed.
+MCRT_HD bool mt(const RayPrep& r, float v0x, float v0y, float v0z, float e1x, float e1y,
+                                   float e1z, float e2x, float e2y, float e2z, float& t, float& u, float& v)
+{
+    float px = (r.dy * e2z) - (r.dz * e2y);
+    float py = (r.dz * e2x) - (r.dx * e2z);
+    float pz = (r.dx * e2y) - (r.dy * e2x);
+    float det = e1x * px + e1y * py + e1z * pz;
+    if (fabsf(det) < 1e-6f) return false;
+    float inv = 1.0f / det;
+    float tx = r.ox - v0x, ty = r.oy - v0y, tz = r.oz - v0z;
+    u = (tx * px + ty * py + tz * pz) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    float qx = (ty * e1z) - (tz * e1y);
+    float qy = (tz * e1x) - (tx * e1z);
+    float qz = (tx * e1y) - (ty * e1x);
+    v = (r.dx * qx + r.dy * qy + r.dz * qz) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+    return true;
+}
+
+struct TraceStack
+{
+    uint32_t* node;  // LDS, [k * stride + lane]
+    float* lo;
+    int stride, lane, depth;
+    uint64_t* spill;  // global, [spill_lane * spill_depth + (k - depth)]
+    int64_t spill_lane;
+    int spill_depth;
+
+    MCRT_HD void push(int sp, uint32_t n, float l)
+    {
+        if (sp < depth)
+        {
+            node[sp * stride + lane] = n;
+            lo[sp * stride + lane] = l;
+        }
+        else
+        {
+            spill[spill_lane * spill_depth + (sp - depth)] =
+                ((uint64_t)bits_of(l) << 32) | (uint64_t)n;
+        }
+    }
+    MCRT_HD void pop(int sp, uint32_t& n, float& l) const
+    {
+        if (sp < depth)
+        {
+            n = node[sp * stride + lane];
+            l = lo[sp * stride + lane];
+        }
+        else
+        {
+            uint64_t x = spill[spill_lane * spill_depth + (sp - depth)];
+            n = (uint32_t)x;
+            l = float_of((uint32_t)(x >> 32));
+        }
+    }
+};
+
+struct TraceCount
+{
+    uint32_t nodes = 0, refs = 0, tris = 0;
+};
+
+template <bool ANY, bool COUNT>
+MCRT_HD int trace(const DevScene& S, TraceStack& st, const RayPrep& r, float tmax,
+                                     float& t_out, float& u_out, float& v_out, TraceCount& cnt)
+{
+    int prim = -1;
+    float R1 = tmax, bu = 0.0f, bv = 0.0f;
+    uint32_t cur = 0;
+    int sp = 0;
+    if (S.nodes == nullptr)
+    {
+        t_out = __builtin_inff();
+        u_out = v_out = 0.0f;
+        return -1;
+    }
+    while (true)
+    {
+        if (!(cur & MCRT_CHILD_LEAF))
+        {
+            const float4* p = reinterpret_cast<const float4*>(S.nodes + cur);
+            const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+            const uint4 ch = reinterpret_cast<const uint4*>(p)[6];
+            const uint4 meta = reinterpret_cast<const uint4*>(p)[7];
+            if (COUNT) cnt.nodes++;
+            float l0, l1, l2, l3;
+            bool p0 = ch.x != MCRT_CHILD_EMPTY && slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, R1, l0);
+            bool p1 = ch.y != MCRT_CHILD_EMPTY && slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, R1, l1);
+            bool p2 = ch.z != MCRT_CHILD_EMPTY && slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, R1, l2);
+            bool p3 = ch.w != MCRT_CHILD_EMPTY && slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, R1, l3);
+            const uint32_t ord = ((r.oct & 4u) ? meta.y : meta.x) >> ((r.oct & 3u) * 8u);
+            uint32_t next = MCRT_CHILD_EMPTY;
+            float next_lo = 0.0f;
+#pragma unroll
+            for (int k = 3; k >= 0; --k)
+            {
+                const uint32_t s = (ord >> (2 * k)) & 3u;
+                const bool ps = s == 0 ? p0 : s == 1 ? p1 : s == 2 ? p2 : p3;
+                if (ps)
+                {
+                    if (next != MCRT_CHILD_EMPTY) { st.push(sp, next, next_lo); ++sp; }
+                    next = s == 0 ? ch.x : s == 1 ? ch.y : s == 2 ? ch.z : ch.w;
+                    next_lo = s == 0 ? l0 : s == 1 ? l1 : s == 2 ? l2 : l3;
+                }
+            }
+            if (next != MCRT_CHILD_EMPTY)
+            {
+                cur = next;
+                continue;
+            }
+        }
+        else
+        {
+            const uint32_t first = cur & MCRT_LEAF_REF_MASK;
+            const uint32_t count = ((cur >> 29) & 3u) + 1u;
+            for (uint32_t j = 0; j < count; ++j)
+            {
+                const float4* q = reinterpret_cast<const float4*>(S.refs + first + j);
+                const float4 a = q[0], b = q[1];
+                if (COUNT) cnt.refs++;
+                float lo;
+                if (!slab(r, a.x, a.w, a.y, b.x, a.z, b.y, R1, lo)) continue;
+                const float4 c = q[2], d = q[3];
+                float t, u, v;
+                if (COUNT) cnt.tris++;
+                if (mt(r, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, t, u, v) && t > r.R0 && t < R1)
+                {
+                    R1 = t;
+                    bu = u;
+                    bv = v;
+                    prim = (int)bits_of(d.w);
+                    if (ANY)
+                    {
+                        t_out = t;
+                        u_out = u;
+                        v_out = v;
+                        return prim;
+                    }
+                }
+            }
+        }
+        bool found = false;
+        while (sp > 0)
+        {
+            --sp;
+            uint32_t n;
+            float lo;
+            st.pop(sp, n, lo);
+            if (!(lo > R1))
+            {
+                cur = n;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    if (prim < 0)
+    {
+        t_out = __builtin_inff();
+        u_out = v_out = 0.0f;
+    }
+    else
+    {
+        t_out = R1;
+        u_out = bu;
+        v_out = bv;
+    }
+    return prim;
+}
+}  // namespace mcrt

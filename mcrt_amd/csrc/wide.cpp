@@ -1,0 +1,200 @@
+// Binary SBVH -> 4-wide nodes.  A wide node is a treelet of up to three
+// binary branches whose frontier (up to four binary nodes) becomes its
+// children; the branch to open next is the one with the largest surface
+// area.  Dropping the treelet's inner boxes is exact: a child box lies
+// inside its parent's box (node box = union of its references' clipped
+// boxes, and every child reference's box lies inside a parent reference's
+// box), and the slab interval of a contained box is contained after
+// rounding, so a child passes only if its parent passes.  Leaf boxes are
+// likewise implied by the boxes of their references.  The visit order of
+// the binary traversal (near child first iff dir[axis] > 0,
+// splitbvh.cpp:73-134) is stored per ray octant.
+#include "wide.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace mcrt
+{
+namespace
+{
+struct TL
+{
+    int axis;
+    int l, r;  // >= 0: treelet node; < 0: slot -(x+1)
+};
+
+struct Builder
+{
+    const BTree& bt;
+    const std::vector<Tri>& tris;
+    WideBVH& out;
+
+    void dfs_slots(const std::vector<TL>& tl, int k, std::vector<int>& seq, const int pos[3]) const
+    {
+        const TL& t = tl[k];
+        int first = pos[t.axis] ? t.l : t.r;
+        int second = pos[t.axis] ? t.r : t.l;
+        for (int x : {first, second})
+        {
+            if (x >= 0) dfs_slots(tl, x, seq, pos);
+            else seq.push_back(-x - 1);
+        }
+    }
+
+    uint32_t leaf_link(const BNode& b)
+    {
+        uint32_t first = (uint32_t)out.refs.size();
+        for (int k = 0; k < b.ref_count; k++)
+        {
+            const SRef& s = bt.refs[b.ref_begin + k];
+            const Tri& t = tris[s.tri];
+            WRef w;
+            for (int a = 0; a < 3; a++)
+            {
+                w.lo[a] = s.box.lo[a];
+                w.hi[a] = s.box.hi[a];
+                w.v0[a] = t.a[a];
+                w.e1[a] = t.b[a] - t.a[a];  // v0v1 (triangle.cpp:16)
+                w.e2[a] = t.c[a] - t.a[a];  // v0v2 (triangle.cpp:17)
+            }
+            w.prim = s.tri;
+            out.refs.push_back(w);
+        }
+        return MCRT_CHILD_LEAF | ((uint32_t)(b.ref_count - 1) << 29) | first;
+    }
+
+    // Returns the wide index and writes the subtree's stack need and depth.
+    int32_t make(const std::vector<int32_t>& frontier_in, const std::vector<TL>& tl_in, int& need,
+                 int& depth)
+    {
+        std::vector<int32_t> slot = frontier_in;
+        std::vector<TL> tl = tl_in;
+        // Open the largest-area branch until four children.
+        while (slot.size() < 4)
+        {
+            int best = -1;
+            float best_a = -1.0f;
+            for (size_t i = 0; i < slot.size(); i++)
+            {
+                const BNode& b = bt.nodes[slot[i]];
+                if (b.leaf) continue;
+                float a = b.box.half_area();
+                if (a > best_a) { best_a = a; best = (int)i; }
+            }
+            if (best < 0) break;
+            const BNode& b = bt.nodes[slot[best]];
+            int ns = (int)slot.size();
+            slot[best] = b.left;
+            slot.push_back(b.right);
+            int tk = (int)tl.size();
+            tl.push_back(TL{b.axis, -(best + 1), -(ns + 1)});
+            for (size_t k = 0; k + 1 < tl.size(); k++)
+            {
+                if (tl[k].l == -(best + 1)) tl[k].l = tk;
+                else if (tl[k].r == -(best + 1)) tl[k].r = tk;
+            }
+        }
+        // Renumber slots in left-first treelet order.
+        std::vector<int> canon;
+        {
+            const int all_pos[3] = {1, 1, 1};
+            dfs_slots(tl, 0, canon, all_pos);
+        }
+        std::vector<int> remap(slot.size());
+        std::vector<int32_t> cslot(slot.size());
+        for (size_t i = 0; i < canon.size(); i++) { remap[canon[i]] = (int)i; cslot[i] = slot[canon[i]]; }
+        for (TL& t : tl)
+        {
+            if (t.l < 0) t.l = -(remap[-t.l - 1] + 1);
+            if (t.r < 0) t.r = -(remap[-t.r - 1] + 1);
+        }
+
+        const int32_t idx = (int32_t)out.nodes.size();
+        out.nodes.emplace_back();
+        WNode w;
+        std::memset(&w, 0, sizeof(w));
+        uint64_t order = 0;
+        for (int o = 0; o < 8; o++)
+        {
+            const int pos[3] = {o & 1, (o >> 1) & 1, (o >> 2) & 1};
+            std::vector<int> seq;
+            dfs_slots(tl, 0, seq, pos);
+            for (int s = 0; s < 4; s++)
+                if (std::find(seq.begin(), seq.end(), s) == seq.end()) seq.push_back(s);
+            uint32_t byte = 0;
+            for (int k = 0; k < 4; k++) byte |= (uint32_t)seq[k] << (2 * k);
+            order |= (uint64_t)byte << (8 * o);
+        }
+        w.order_lo = (uint32_t)order;
+        w.order_hi = (uint32_t)(order >> 32);
+
+        int nchild = 0, child_need = 0, child_depth = 0;
+        for (int s = 0; s < 4; s++)
+        {
+            w.child[s] = MCRT_CHILD_EMPTY;
+            if (s >= (int)cslot.size()) continue;
+            const BNode& b = bt.nodes[cslot[s]];
+            if (b.leaf && b.ref_count == 0) continue;  // cannot produce a hit
+            w.lo_x[s] = b.box.lo[0]; w.hi_x[s] = b.box.hi[0];
+            w.lo_y[s] = b.box.lo[1]; w.hi_y[s] = b.box.hi[1];
+            w.lo_z[s] = b.box.lo[2]; w.hi_z[s] = b.box.hi[2];
+            nchild++;
+            if (b.leaf)
+            {
+                w.child[s] = leaf_link(b);
+                continue;
+            }
+            int cn = 0, cd = 0;
+            std::vector<TL> ctl{TL{b.axis, -1, -2}};
+            w.child[s] = (uint32_t)make({b.left, b.right}, ctl, cn, cd);
+            child_need = std::max(child_need, cn);
+            child_depth = std::max(child_depth, cd);
+        }
+        w.nchild = (uint32_t)nchild;
+        out.nodes[idx] = w;
+        // The kernel pushes every passing child and pops one.
+        need = std::max(nchild, nchild - 1 + child_need);
+        depth = 1 + child_depth;
+        return idx;
+    }
+};
+}  // namespace
+
+void collapse_wide(const BTree& bt, const std::vector<Tri>& tris, WideBVH& out)
+{
+    out.nodes.clear();
+    out.refs.clear();
+    out.stack_bound = 0;
+    out.depth = 0;
+    if (bt.nodes.empty()) return;
+    Builder b{bt, tris, out};
+    const BNode& root = bt.nodes[0];
+    int need = 0, depth = 0;
+    if (root.leaf)
+    {
+        WNode w;
+        std::memset(&w, 0, sizeof(w));
+        for (int s = 0; s < 4; s++) w.child[s] = MCRT_CHILD_EMPTY;
+        w.order_lo = w.order_hi = 0xe4e4e4e4u;  // 0,1,2,3 for every octant
+        out.nodes.push_back(w);
+        if (root.ref_count > 0)
+        {
+            WNode& r = out.nodes[0];
+            r.lo_x[0] = root.box.lo[0]; r.hi_x[0] = root.box.hi[0];
+            r.lo_y[0] = root.box.lo[1]; r.hi_y[0] = root.box.hi[1];
+            r.lo_z[0] = root.box.lo[2]; r.hi_z[0] = root.box.hi[2];
+            uint32_t link = b.leaf_link(root);
+            out.nodes[0].child[0] = link;
+            out.nodes[0].nchild = 1;
+        }
+        out.stack_bound = 1;
+        out.depth = 1;
+        return;
+    }
+    std::vector<TL> tl{TL{root.axis, -1, -2}};
+    b.make({root.left, root.right}, tl, need, depth);
+    out.stack_bound = need;
+    out.depth = depth;
+}
+}  // namespace mcrt

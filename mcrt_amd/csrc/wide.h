@@ -1,0 +1,19 @@
+// Collapse of the binary SBVH into 4-wide nodes (WNode/WRef, layout.h).
+#pragma once
+#include <vector>
+
+#include "layout.h"
+#include "sbvh.h"
+
+namespace mcrt
+{
+struct WideBVH
+{
+    std::vector<WNode> nodes;  // nodes[0] = root (absent if no triangles)
+    std::vector<WRef> refs;
+    int32_t stack_bound = 0;   // stack entries a traversal can ever hold
+    int32_t depth = 0;         // wide levels
+};
+
+void collapse_wide(const BTree& bt, const std::vector<Tri>& tris, WideBVH& out);
+}  // namespace mcrt
