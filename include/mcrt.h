@@ -109,6 +109,10 @@ MCRT_API int mcrt_scene_destroy(mcrt_scene* scene);
 /* Material(c, e, t) + todo_addMaterial: returns the material id. */
 MCRT_API int mcrt_material_add(mcrt_scene* scene, int type, const float rgb[3], float glossiness,
                                int32_t* out_id);
+/* n materials at once (types, 3 floats rgb, glossiness per material); ids
+ * are first_id .. first_id + n - 1. */
+MCRT_API int mcrt_materials_add(mcrt_scene* scene, size_t n, const int32_t* types, const float* rgb,
+                                const float* glossiness, int32_t* first_id);
 /* Bulk replacement of material colours (albedo / emission) after finalize,
  * for optimisation loops; rgb holds 3 floats per material id. */
 MCRT_API int mcrt_material_set_rgb(mcrt_scene* scene, const float* rgb, size_t nmat);

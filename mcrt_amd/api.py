@@ -68,6 +68,7 @@ def lib():
         "mcrt_scene_create": (I, [P, ctypes.POINTER(P)]),
         "mcrt_scene_destroy": (I, [P]),
         "mcrt_material_add": (I, [P, I, P, ctypes.c_float, ctypes.POINTER(ctypes.c_int32)]),
+        "mcrt_materials_add": (I, [P, Z, P, P, P, ctypes.POINTER(ctypes.c_int32)]),
         "mcrt_material_set_rgb": (I, [P, P, Z]),
         "mcrt_scene_add_triangles": (I, [P, P, P, P, Z]),
         "mcrt_scene_load_obj": (I, [P, ctypes.c_char_p, P, ctypes.c_int32, ctypes.POINTER(Z)]),
@@ -153,6 +154,16 @@ class Scene:
         out = ctypes.c_int32()
         _check(lib().mcrt_material_add(self.h, int(type_), rgb.ctypes.data, float(glossiness), ctypes.byref(out)))
         return out.value
+
+    def add_materials(self, types, rgb, gloss):
+        """n materials at once; returns the first id."""
+        types = np.ascontiguousarray(types, np.int32).reshape(-1)
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+        gloss = np.ascontiguousarray(np.broadcast_to(np.asarray(gloss, np.float32), types.shape))
+        first = ctypes.c_int32()
+        _check(lib().mcrt_materials_add(self.h, types.shape[0], types.ctypes.data, rgb.ctypes.data,
+                                        gloss.ctypes.data, ctypes.byref(first)))
+        return first.value
 
     def set_material_rgb(self, rgb):
         rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)

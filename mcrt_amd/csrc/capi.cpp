@@ -154,6 +154,31 @@ MCRT_API int mcrt_material_add(mcrt_scene* s, int type, const float rgb[3], floa
     return MCRT_OK;
 }
 
+MCRT_API int mcrt_materials_add(mcrt_scene* s, size_t n, const int32_t* types, const float* rgb,
+                                const float* gloss, int32_t* first_id)
+{
+    if (!s || (n && (!types || !rgb || !gloss))) return fail(MCRT_ERR_INVALID_ARG, "mcrt_materials_add: NULL argument");
+    if (s->finalized) return fail(MCRT_ERR_STATE, "mcrt_materials_add: scene already finalized");
+    for (size_t i = 0; i < n; i++)
+    {
+        int t = types[i];
+        if (t != MCRT_LGHT && t != MCRT_DIFF && t != MCRT_GLSY)
+            return fail(MCRT_ERR_INVALID_ARG, "mcrt_materials_add: unsupported material type %d at %zu", t, i);
+        if (t == MCRT_GLSY && !(gloss[i] >= 0.0f))
+            return fail(MCRT_ERR_INVALID_ARG, "mcrt_materials_add: glossiness must be >= 0 (material %zu)", i);
+    }
+    if (first_id) *first_id = (int32_t)s->mats.size();
+    for (size_t i = 0; i < n; i++)
+    {
+        HostMaterial m;
+        m.type = types[i];
+        m.rgb[0] = rgb[3 * i]; m.rgb[1] = rgb[3 * i + 1]; m.rgb[2] = rgb[3 * i + 2];
+        m.gloss = gloss[i];
+        s->mats.push_back(m);
+    }
+    return MCRT_OK;
+}
+
 MCRT_API int mcrt_material_set_rgb(mcrt_scene* s, const float* rgb, size_t nmat)
 {
     if (!s || (!rgb && nmat)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_material_set_rgb: NULL argument");

@@ -1,0 +1,282 @@
+"""Synthetic scenes and ray sets of the benchmark configurations (numpy,
+seeded, deterministic).  All geometry is produced in float64 and rounded
+to float32 once at the end.
+
+Configurations:
+  * cornell_icosphere(sub=4): Cornell box (5 diffuse quads = 10 triangles,
+    albedo 0.75 grey, red and green side walls) + 2-triangle ceiling light
+    (emission 15) + centred icosphere (albedo 0.8) -- 5,132 triangles.
+  * cornell_displaced(sub=8): the same box and light around an icosphere
+    of subdivision 8 (1,310,720 triangles) radially displaced by 3-octave
+    value noise (amplitude 5% of the radius, seed 7), per-triangle albedo
+    U[0.2, 0.9] (seed 11), 10% of triangles Phong with exponent 64.
+  * soup(n): triangle soup, centres U[0,1]^3, edge directions uniform on
+    the sphere, edge lengths U[0.002, 0.01] (seed 3).
+"""
+import numpy as np
+
+from . import api
+
+# The Cornell box of the reference demo at its x0.1 scale
+# (src/examples/pt/demo_pt.cpp:83-203): vertex order gives inward normals.
+_S = 0.1
+_WALLS = {
+    "red": [(556, 0, 0), (556, 0, 559.20001), (556, 548.79999, 559.20001), (556, 548.79999, 0)],
+    "green": [(0, 0, 559.20001), (0, 0, 0), (0, 548.79999, 0), (0, 548.79999, 559.20001)],
+    "floor": [(556, 0, 0), (0, 0, 0), (0, 0, 559.20001), (556, 0, 559.20001)],
+    "back": [(556, 0, 559.20001), (0, 0, 559.20001), (0, 548.79999, 559.20001), (556, 548.79999, 559.20001)],
+    "ceiling": [(556, 548.79999, 0), (556, 548.79999, 559.20001), (0, 548.79999, 559.20001), (0, 548.79999, 0)],
+}
+_LIGHT_Y = np.float32(548.79999) - np.float32(0.5)
+_LIGHT = [(343, _LIGHT_Y, 227), (343, _LIGHT_Y, 332), (213, _LIGHT_Y, 332), (213, _LIGHT_Y, 227)]
+
+CAMERA = dict(origin=(27.8, 27.3, -80.0), target=(27.8, 27.3, -79.9), fov_y=39.3077)
+BOX_CENTER = np.array([27.8, 27.439999, 27.96], np.float64)
+SPHERE_RADIUS = 12.0
+
+
+def _quad(p):
+    p = np.asarray(p, np.float32) * np.float32(_S)
+    return np.stack([np.stack([p[0], p[1], p[2]]), np.stack([p[0], p[2], p[3]])])
+
+
+def cornell_box():
+    """(tris (12,3,3) float32, kind list): 10 wall triangles + 2 light triangles.
+
+    Light first (as the reference adds it first), then red, green, floor,
+    back, ceiling."""
+    tris = [_quad(_LIGHT)]
+    kinds = ["light", "light"]
+    for name in ("red", "green", "floor", "back", "ceiling"):
+        tris.append(_quad(_WALLS[name]))
+        kinds += [name, name]
+    return np.concatenate(tris).astype(np.float32), kinds
+
+
+def icosphere(sub):
+    """Unit icosphere: (verts (V,3) float64, faces (20*4^sub,3) int64), outward winding."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    for _ in range(sub):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = v.shape[0]
+        v = np.concatenate([v, mid])
+        nf = f.shape[0]
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        ab, bc, ca = base + inv[:nf], base + inv[nf:2 * nf], base + inv[2 * nf:]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                            np.stack([ab, bc, ca], 1)])
+    return v, f
+
+
+def value_noise3(p, octaves=3, seed=7, lattice=32, base_freq=2.0):
+    """3-octave value noise in [-1, 1] at points p (N,3) in [-1,1]^3.
+
+    Lattice of uniform values (seed) with trilinear interpolation of
+    smoothstep weights; octave k has frequency base_freq*2^k and weight
+    0.5^k, normalised by the weight sum."""
+    rng = np.random.default_rng(seed)
+    g = rng.uniform(-1.0, 1.0, (lattice, lattice, lattice))
+    out = np.zeros(p.shape[0])
+    wsum = 0.0
+    for k in range(octaves):
+        q = (p + 1.0) * (base_freq * 2 ** k)
+        i0 = np.floor(q).astype(np.int64)
+        fr = q - i0
+        w = fr * fr * (3.0 - 2.0 * fr)
+        acc = 0.0
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    ix = (i0[:, 0] + dx) % lattice
+                    iy = (i0[:, 1] + dy) % lattice
+                    iz = (i0[:, 2] + dz) % lattice
+                    wt = (w[:, 0] if dx else 1 - w[:, 0]) * (w[:, 1] if dy else 1 - w[:, 1]) * \
+                         (w[:, 2] if dz else 1 - w[:, 2])
+                    acc = acc + wt * g[ix, iy, iz]
+        out += acc * 0.5 ** k
+        wsum += 0.5 ** k
+    return out / wsum
+
+
+def vertex_normals(v, f):
+    """Area-weighted vertex normals (the Mesh::genNormals rule) in float64."""
+    fn = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    vn = np.zeros_like(v)
+    for k in range(3):
+        np.add.at(vn, f[:, k], fn)
+    return vn / np.linalg.norm(vn, axis=1, keepdims=True)
+
+
+class SceneData:
+    """Triangles + per-corner normals + per-triangle material index + material table."""
+
+    def __init__(self, tris, normals, tri_mat, mat_type, mat_rgb, mat_gloss):
+        self.tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
+        self.tri_mat = np.ascontiguousarray(tri_mat, np.int32)
+        self.mat_type = np.ascontiguousarray(mat_type, np.int32)
+        self.mat_rgb = np.ascontiguousarray(mat_rgb, np.float32).reshape(-1, 3)
+        self.mat_gloss = np.ascontiguousarray(mat_gloss, np.float32)
+
+    @property
+    def ntri(self):
+        return self.tris.shape[0]
+
+    def to_scene(self, ctx, width=None, height=None, camera=CAMERA, finalize=True, threads=0):
+        """Builds an mcrt Scene (materials, triangles, camera) and finalizes it."""
+        s = api.Scene(ctx)
+        s.add_materials(self.mat_type, self.mat_rgb, self.mat_gloss)
+        s.add(self.tris, self.tri_mat, self.normals)
+        if width:
+            s.set_camera(camera["origin"], camera["target"], width, height, camera["fov_y"])
+        if finalize:
+            s.finalize(threads)
+        return s
+
+
+def _box_materials():
+    grey, red, green = (0.75, 0.75, 0.75), (0.75, 0.25, 0.25), (0.25, 0.75, 0.25)
+    return grey, red, green
+
+
+def cornell_icosphere(sub=4, radius=SPHERE_RADIUS, albedo=0.8, emission=15.0):
+    """Cornell box + light + centred icosphere (smooth vertex normals)."""
+    box, kinds = cornell_box()
+    v, f = icosphere(sub)
+    vn = vertex_normals(v, f)
+    sv = BOX_CENTER + radius * v
+    sph = sv[f].astype(np.float32)
+    sph_n = vn[f].astype(np.float32)
+    grey, red, green = _box_materials()
+    # materials: 0 light, 1 grey, 2 red, 3 green, 4 sphere
+    mat_type = [api.LGHT, api.DIFF, api.DIFF, api.DIFF, api.DIFF]
+    mat_rgb = [(emission,) * 3, grey, red, green, (albedo,) * 3]
+    kind_mat = {"light": 0, "red": 2, "green": 3, "floor": 1, "back": 1, "ceiling": 1}
+    box_mat = [kind_mat[k] for k in kinds]
+    box_n = np.repeat(_face_normals(box)[:, None, :], 3, axis=1)
+    tris = np.concatenate([box, sph])
+    normals = np.concatenate([box_n, sph_n])
+    tri_mat = np.concatenate([np.array(box_mat, np.int32), np.full(len(f), 4, np.int32)])
+    return SceneData(tris, normals, tri_mat, mat_type, mat_rgb, [25.0] * len(mat_type))
+
+
+def _face_normals(t):
+    """normalize(cross(a-c, b-c)) in float32 (triangle.cpp:67-74)."""
+    t = np.asarray(t, np.float32)
+    n = np.cross(t[:, 0] - t[:, 2], t[:, 1] - t[:, 2]).astype(np.float32)
+    return (n * (np.float32(1.0) / np.sqrt((n * n).sum(1, dtype=np.float32)))[:, None]).astype(np.float32)
+
+
+def cornell_displaced(sub=8, radius=SPHERE_RADIUS, amp=0.05, noise_seed=7, albedo_seed=11, phong_frac=0.1,
+                      phong_exp=64.0, emission=15.0, albedo_range=(0.2, 0.9), per_tri_albedo=True):
+    """The at-scale configuration: displaced icosphere with per-triangle materials."""
+    box, kinds = cornell_box()
+    v, f = icosphere(sub)
+    disp = 1.0 + amp * value_noise3(v, seed=noise_seed)
+    sv = BOX_CENTER + radius * v * disp[:, None]
+    vn = vertex_normals(sv, f)
+    sph = sv[f].astype(np.float32)
+    sph_n = vn[f].astype(np.float32)
+    ns = len(f)
+    rng = np.random.default_rng(albedo_seed)
+    alb = rng.uniform(albedo_range[0], albedo_range[1], ns)
+    phong = rng.uniform(0.0, 1.0, ns) < phong_frac
+    grey, red, green = _box_materials()
+    kind_mat = {"light": 0, "red": 2, "green": 3, "floor": 1, "back": 1, "ceiling": 1}
+    box_mat = np.array([kind_mat[k] for k in kinds], np.int32)
+    if per_tri_albedo:
+        mat_type = np.concatenate([[api.LGHT, api.DIFF, api.DIFF, api.DIFF],
+                                   np.where(phong, api.GLSY, api.DIFF)]).astype(np.int32)
+        mat_rgb = np.concatenate([np.array([(emission,) * 3, grey, red, green]),
+                                  np.repeat(alb[:, None], 3, axis=1)])
+        sph_mat = 4 + np.arange(ns, dtype=np.int32)
+    else:
+        mat_type = np.array([api.LGHT, api.DIFF, api.DIFF, api.DIFF, api.DIFF], np.int32)
+        mat_rgb = np.array([(emission,) * 3, grey, red, green, (0.8,) * 3])
+        sph_mat = np.full(ns, 4, np.int32)
+    mat_gloss = np.where(mat_type == api.GLSY, phong_exp, 25.0)
+    box_n = np.repeat(_face_normals(box)[:, None, :], 3, axis=1)
+    tris = np.concatenate([box, sph])
+    normals = np.concatenate([box_n, sph_n])
+    tri_mat = np.concatenate([box_mat, sph_mat])
+    return SceneData(tris, normals, tri_mat, mat_type, mat_rgb, mat_gloss)
+
+
+def soup(n=1 << 22, seed=3, lmin=0.002, lmax=0.01):
+    """Triangle soup (n,9) float32."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, 1.0, (n, 3))
+    d1 = _sphere_dirs(rng, n)
+    d2 = _sphere_dirs(rng, n)
+    l1 = rng.uniform(lmin, lmax, (n, 1))
+    l2 = rng.uniform(lmin, lmax, (n, 1))
+    v0 = c - (l1 * d1 + l2 * d2) / 3.0
+    return np.concatenate([v0, v0 + l1 * d1, v0 + l2 * d2], axis=1).astype(np.float32)
+
+
+def _sphere_dirs(rng, n):
+    z = 1.0 - 2.0 * rng.uniform(0.0, 1.0, n)
+    phi = 2.0 * np.pi * rng.uniform(0.0, 1.0, n)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def incoherent_rays(n, lo, hi, seed=3, tmin=1e-4, tmax=np.inf):
+    """Rays (n,8) float32: origins U[lo,hi]^3, directions uniform on the sphere."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(0.0, 1.0, (n, 3)) * (np.asarray(hi, np.float64) - lo) + lo
+    d = _sphere_dirs(rng, n)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.empty((n, 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = tmin
+    r[:, 4:7] = d
+    r[:, 7] = tmax
+    return r
+
+
+def pinhole_rays(width, height, origin=(0.5, 0.5, -1.5), target=(0.5, 0.5, 0.5), fov_y=45.0, tmin=1e-4,
+                 tmax=np.inf):
+    """Pinhole grid through pixel centres (Camera::makeRay, camera.cpp:37-43)."""
+    o = np.asarray(origin, np.float64)
+    fwd = np.asarray(target, np.float64) - o
+    fwd /= np.linalg.norm(fwd)
+    fx = np.cross(fwd, [0.0, 1.0, 0.0])
+    fx /= np.linalg.norm(fx)
+    fy = np.cross(fx, fwd)
+    dist = 1.0 / (2.0 * np.tan(np.radians(fov_y * 0.5)))
+    ww = width / height
+    xs = (np.arange(width) + 0.5) / width - 0.5
+    ys = (np.arange(height) + 0.5) / height - 0.5
+    X, Y = np.meshgrid(xs, ys)
+    d = fx[None, None, :] * (X[..., None] * ww) + fy[None, None, :] * Y[..., None] + fwd * dist
+    d = d.reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.empty((d.shape[0], 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = tmin
+    r[:, 4:7] = d
+    r[:, 7] = tmax
+    return r
+
+
+def write_obj(path, tris):
+    """Writes triangles as an OBJ of `v` and `f a b c` lines (no vertex sharing)."""
+    t = np.asarray(tris, np.float32).reshape(-1, 3)
+    with open(path, "w") as fh:
+        for p in t:
+            fh.write("v %.9g %.9g %.9g\n" % (p[0], p[1], p[2]))
+        n = t.shape[0] // 3
+        for i in range(n):
+            fh.write("f %d %d %d\n" % (3 * i + 1, 3 * i + 2, 3 * i + 3))

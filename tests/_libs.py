@@ -1,0 +1,161 @@
+"""ctypes handles of the test-side libraries: the oracle (oracle/liboracle.so),
+the compiled reference harness (oracle/_ref/libmcrt_ref.so, optional) and the
+host emulation of the product's query pipeline (tests/hostemu/libhostemu.so).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+P, I, LL, U64, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_uint64, ctypes.c_float
+
+_cache = {}
+
+
+def _load(path, sigs):
+    if path in _cache:
+        return _cache[path]
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _cache[path] = L
+    return L
+
+
+def oracle():
+    return _load(os.path.join(ROOT, "oracle", "liboracle.so"), {
+        "orc_ray_box": (I, [P, P]),
+        "orc_ray_tri": (I, [P, P, P]),
+        "orc_clip_bound_tight": (None, [P, P, F, F, I, P]),
+        "orc_bvh_build": (P, [P, I]),
+        "orc_bvh_free": (None, [P]),
+        "orc_bvh_digest": (U64, [P, P]),
+        "orc_bvh_intersect": (None, [P, P, LL, P, P, P, I]),
+        "orc_brute_intersect": (None, [P, I, P, LL, P, P, I]),
+        "orc_load_obj": (LL, [ctypes.c_char_p, P, P]),
+        "orc_num_procs": (I, []),
+        "orc_scene_create": (P, [P, P, P, I, P, P, P, I]),
+        "orc_scene_free": (None, [P]),
+        "orc_scene_bvh": (P, [P]),
+        "orc_render": (None, [P, P, P, U64, LL, P, P, I]),
+        "orc_camera_rays": (None, [P, I, I, P, LL, P]),
+        "orc_trace_fixed": (None, [P, P, P, U64, P, LL, P, I]),
+        "orc_mix64": (U64, [U64]),
+        "orc_rng_u01": (F, [U64, U64, U64, ctypes.c_uint32, ctypes.c_uint32]),
+        "orc_own_sincos": (None, [P, LL, P, P]),
+        "orc_own_pow": (None, [P, P, LL, P]),
+    })
+
+
+def reference():
+    return _load(os.path.join(ROOT, "oracle", "_ref", "libmcrt_ref.so"), {
+        "ref_bvh_build": (P, [P, I]),
+        "ref_bvh_free": (None, [P]),
+        "ref_bvh_digest": (U64, [P, P]),
+        "ref_bvh_intersect": (None, [P, P, LL, P, P]),
+        "ref_scenebvh_query": (None, [P, I, P, LL, F, P, P, P, P]),
+        "ref_debug_mesh": (I, [P]),
+        "ref_ray_box": (I, [P, P]),
+        "ref_ray_tri": (I, [P, P, P]),
+        "ref_clip_bound_tight": (None, [P, P, F, F, I, P]),
+        "ref_load_obj": (LL, [ctypes.c_char_p, P, P, P]),
+        "ref_camera_rays": (None, [P, P, I, I, F, P, LL, P]),
+        "ref_bsdf": (None, [I, P, P, LL, P]),
+        "ref_discrete": (None, [P, I, P, LL, P, P]),
+    })
+
+
+def hostemu():
+    return _load(os.path.join(ROOT, "tests", "hostemu", "libhostemu.so"), {
+        "emu_build": (P, [P, LL, I]),
+        "emu_digest": (U64, [P, P]),
+        "emu_free": (None, [P]),
+        "emu_trace": (None, [P, P, LL, P, P, I, I, P, I]),
+    })
+
+
+def ptr(a):
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------- helpers --
+class OracleBVH:
+    def __init__(self, tris):
+        self.tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        self.L = oracle()
+        self.h = self.L.orc_bvh_build(ptr(self.tris), self.tris.shape[0])
+
+    def digest(self):
+        st = np.zeros(5, np.int64)
+        d = self.L.orc_bvh_digest(self.h, ptr(st))
+        return d, st
+
+    def intersect(self, rays, threads=8, counters=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        prim = np.empty(n, np.int32)
+        tuv = np.empty((n, 3), np.float32)
+        cnt = np.zeros(3, np.int64)
+        self.L.orc_bvh_intersect(self.h, ptr(rays), n, ptr(prim), ptr(tuv), ptr(cnt) if counters else None,
+                                 threads)
+        return (prim, tuv, cnt) if counters else (prim, tuv)
+
+    def __del__(self):
+        try:
+            self.L.orc_bvh_free(self.h)
+        except Exception:
+            pass
+
+
+def brute(tris, rays, threads=8):
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, np.float32)
+    n = rays.shape[0]
+    prim = np.empty(n, np.int32)
+    tuv = np.empty((n, 3), np.float32)
+    oracle().orc_brute_intersect(ptr(tris), tris.shape[0], ptr(rays), n, ptr(prim), ptr(tuv), threads)
+    return prim, tuv
+
+
+def debug_mesh():
+    """The reference's 968-triangle DebugMesh (fixture file, or the compiled reference)."""
+    path = os.path.join(ROOT, "tests", "golden", "debug_mesh.npz")
+    if os.path.exists(path):
+        return np.load(path)["tris"]
+    R = reference()
+    n = R.ref_debug_mesh(None)
+    t = np.zeros((n, 9), np.float32)
+    R.ref_debug_mesh(ptr(t))
+    return t
+
+
+def random_rays(tris, n, seed=0, margin=0.2, tmin=1e-4, tmax=np.inf):
+    """Rays with origins in the (expanded) bounds of tris and directions
+    uniform on the sphere (float32)."""
+    rng = np.random.default_rng(seed)
+    p = np.asarray(tris, np.float32).reshape(-1, 3)
+    lo, hi = p.min(0).astype(np.float64), p.max(0).astype(np.float64)
+    ext = hi - lo
+    o = rng.uniform(lo - margin * ext, hi + margin * ext, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.empty((n, 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = tmin
+    r[:, 4:7] = d
+    r[:, 7] = tmax
+    return r
+
+
+def soup_slice(n_total=1 << 22, seed=3, lo=0.0, hi=0.25):
+    """Triangles of the 2^22 soup whose centroid lies in [lo,hi)^3."""
+    from mcrt_amd import scenes
+    t = scenes.soup(n_total, seed)
+    c = t.reshape(-1, 3, 3).mean(axis=1)
+    keep = np.all((c >= lo) & (c < hi), axis=1)
+    return np.ascontiguousarray(t[keep])

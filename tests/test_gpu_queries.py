@@ -1,0 +1,136 @@
+"""GPU parity of mcrt_intersect / mcrt_occluded against the oracle's
+restatement of SplitBVH traversal (bit-exact primitive ids and t/u/v).
+
+Reference behaviour: SplitBVH::intersect (splitbvh.cpp:5-145, :1627-1642);
+SceneBVH::occluded (scenebvh.cpp:379-396)."""
+import numpy as np
+import pytest
+
+import _libs
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(ctx, tris):
+    import mcrt_amd
+    s = mcrt_amd.Scene(ctx)
+    m = s.add_material(mcrt_amd.DIFF, (0.5, 0.5, 0.5))
+    s.add(tris, m)
+    st = s.finalize()
+    return s, st
+
+
+def _check_closest(ctx, tris, rays, label):
+    import mcrt_amd
+    print(f"[{label}] ntri={len(tris)} nrays={len(rays)}", flush=True)
+    s, st = _scene(ctx, tris)
+    ob = _libs.OracleBVH(tris)
+    d, _ = ob.digest()
+    assert st["tree_digest"] == d, "product SBVH differs from the oracle's"
+    hits = s.intersect(rays)
+    prim, tuv = mcrt_amd.split_hits(hits)
+    oprim, otuv = ob.intersect(rays)
+    bad = np.nonzero(prim != oprim)[0]
+    assert bad.size == 0, f"{bad.size} primitive mismatches, first ray {bad[:5]}"
+    assert np.array_equal(tuv.view(np.uint32), otuv.view(np.uint32)), "t/u/v differ bitwise"
+    return s, st, oprim
+
+
+def test_debug_mesh_closest_and_any(ctx):
+    tris = _libs.debug_mesh()
+    rays = _libs.random_rays(tris, 1 << 16, seed=1)
+    s, st, oprim = _check_closest(ctx, tris, rays, "debug_mesh")
+    assert st["stack_bound"] > 16, "fixture should exercise the stack spill path"
+    # any-hit with finite tmax = occluded(tfar) semantics
+    r2 = rays.copy()
+    r2[:, 7] = np.float32(0.5) - np.float32(1e-4)
+    occ = s.occluded(r2)
+    p2, _ = _libs.OracleBVH(tris).intersect(r2)
+    assert np.array_equal(occ, (p2 >= 0).astype(np.uint8))
+
+
+def test_cornell_icosphere(ctx):
+    from mcrt_amd import scenes
+    d = scenes.cornell_icosphere(4)
+    lo = np.array([0.1, 0.1, 0.1]) * 1.0
+    rays = scenes.incoherent_rays(1 << 16, lo=np.array([1.0, 1.0, 1.0]), hi=np.array([54.0, 53.0, 54.0]), seed=5)
+    _check_closest(ctx, d.tris, rays, "cornell_ico4")
+
+
+def test_soup_slice(ctx):
+    tris = _libs.soup_slice()
+    rays = _libs.random_rays(tris, 1 << 16, seed=2, margin=0.0)
+    _check_closest(ctx, tris, rays, "soup_slice")
+
+
+def test_axis_aligned_rays_miss_like_reference(ctx):
+    # A zero direction component gives inv_dir = 0 and the slab test rejects
+    # every box (aabb.cpp:128-191 with ray.cpp:77-79): such rays miss.
+    tris = _libs.debug_mesh()
+    rays = _libs.random_rays(tris, 4096, seed=3)
+    rays[:, 4:7] = 0.0
+    rays[:, 4 + (np.arange(4096) % 3)] = np.where(np.arange(4096) % 2 == 0, 1.0, -1.0)
+    _check_closest(ctx, tris, rays, "axis_aligned")
+
+
+def test_tmax_and_tmin_bounds(ctx):
+    tris = _libs.debug_mesh()
+    rays = _libs.random_rays(tris, 1 << 14, seed=4)
+    ob = _libs.OracleBVH(tris)
+    p, tuv = ob.intersect(rays)
+    hit = p >= 0
+    # tmax exactly at the hit distance must exclude it (strict t < tmax)
+    r = rays.copy()
+    r[hit, 7] = tuv[hit, 0]
+    _check_closest(ctx, tris, r, "tmax_eq_t")
+    # tmin exactly at the hit distance must exclude it (strict t > tmin)
+    r = rays.copy()
+    r[hit, 3] = tuv[hit, 0]
+    _check_closest(ctx, tris, r, "tmin_eq_t")
+
+
+def test_device_pointer_path(ctx):
+    import torch
+    import mcrt_amd
+    tris = _libs.debug_mesh()
+    rays = _libs.random_rays(tris, 1 << 15, seed=6)
+    s, _ = _scene(ctx, tris)
+    dr = torch.from_numpy(rays).cuda()
+    dh = torch.empty((rays.shape[0], 4), dtype=torch.int32, device="cuda")
+    s.intersect(dr, dh)
+    do = torch.empty(rays.shape[0], dtype=torch.uint8, device="cuda")
+    s.occluded(dr, do)
+    torch.cuda.synchronize()
+    prim, tuv = mcrt_amd.split_hits(dh.cpu().numpy())
+    op, otuv = _libs.OracleBVH(tris).intersect(rays)
+    assert np.array_equal(prim, op)
+    assert np.array_equal(tuv.view(np.uint32), otuv.view(np.uint32))
+    assert np.array_equal(do.cpu().numpy(), (op >= 0).astype(np.uint8))
+
+
+def test_empty_scene_and_zero_rays(ctx):
+    import mcrt_amd
+    s = mcrt_amd.Scene(ctx)
+    st = s.finalize()
+    assert st["triangles"] == 0
+    rays = _libs.random_rays(np.zeros((1, 9), np.float32) + np.arange(9, dtype=np.float32), 256, seed=7)
+    prim, tuv = mcrt_amd.split_hits(s.intersect(rays))
+    assert (prim == -1).all() and np.isinf(tuv[:, 0]).all()
+    assert (s.occluded(rays) == 0).all()
+    s2, _ = _scene(ctx, _libs.debug_mesh())
+    out = s2.intersect(np.zeros((0, 8), np.float32))
+    assert out.shape == (0, 4)
+
+
+def test_errors(ctx):
+    import mcrt_amd
+    s = mcrt_amd.Scene(ctx)
+    with pytest.raises(mcrt_amd.McrtError) as e:
+        s.intersect(np.zeros((4, 8), np.float32))
+    assert e.value.code == 2  # query before finalize
+    with pytest.raises(mcrt_amd.McrtError) as e:
+        s.add(np.zeros((1, 9), np.float32), 7)
+    assert e.value.code == 1  # unknown material
+    with pytest.raises(mcrt_amd.McrtError) as e:
+        s.load_model("/nonexistent/file.obj", s.add_material(mcrt_amd.DIFF, (1, 1, 1)))
+    assert e.value.code == 3
