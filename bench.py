@@ -1,0 +1,178 @@
+"""Benchmark of the MI355X ray-query core and path tracer on the
+at-scale configuration: Cornell box + light around an icosphere of
+subdivision 8 (1,310,720 triangles) radially displaced by 3-octave value
+noise, per-triangle albedo, 10% Phong (exponent 64); 16,777,216 incoherent
+rays (origins uniform in the box, directions uniform on the sphere, seed 3).
+
+One step = one closest-hit pass over the 16M rays, resident in HBM.
+Prints one JSON line: value = Mrays/s closest-hit (whole job), plus the
+roofline of the traversal kernel and the CPU baseline (the oracle's
+restatement of the reference traversal on the host cores).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+NODE_B, REF_B, TRI_B, RAY_B, HIT_B = 128, 32, 32, 32, 16
+BOX_LO = np.array([0.5, 0.5, 0.5])
+BOX_HI = np.array([55.1, 54.3, 55.4])
+
+
+PROFILE = os.path.join(ROOT, "profiles", "k_trace_closest_cornell_ico8_16M.md")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def profiled_traffic():
+    """HBM bytes per traversal launch measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE scaled by a copy-kernel calibration) for this workload."""
+    try:
+        with open(PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(scene_data, rays_np, sample):
+    """Oracle (restated reference traversal, scalar recursive) on `sample`
+    rays with every host core."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _libs
+    L = _libs.oracle()
+    if L is None:
+        return None
+    cores = L.orc_num_procs()
+    t0 = time.time()
+    ob = _libs.OracleBVH(scene_data.tris)
+    tb = time.time() - t0
+    sub = np.ascontiguousarray(rays_np[:sample])
+    ob.intersect(sub[: min(65536, sample)], threads=cores)  # warm-up
+    t0 = time.time()
+    prim, _ = ob.intersect(sub, threads=cores)
+    dt = time.time() - t0
+    return {"value": round(sample / dt / 1e6, 3), "unit": "Mrays/s", "cores": int(cores), "kind": "port",
+            "sample": f"closest-hit of the first {sample} of the same rays, oracle SplitBVH build "
+                      f"({tb:.1f} s, excluded) + recursive reference-order traversal, OpenMP over rays",
+            "hit_fraction": round(float((prim >= 0).mean()), 4)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=int, default=1 << 24)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22)
+    ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(every GPU box has one GPU).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if args.copy_calib:
+        nbytes = 1 << 30
+        src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_copy": nbytes, "copies": 3}), flush=True)
+        return
+
+    import mcrt_amd
+    from mcrt_amd import scenes
+
+    t0 = time.time()
+    data = scenes.cornell_displaced(8)
+    log(f"scene: {data.ntri} triangles generated in {time.time() - t0:.1f} s")
+    ctx = mcrt_amd.Context(0)
+    scene = data.to_scene(ctx)
+    st = scene.stats
+    log(f"SBVH: {st['build_seconds']:.2f} s build, {st['refs']} refs, {st['wide_nodes']} wide nodes")
+
+    n = args.rays
+    rays_np = scenes.incoherent_rays(n, lo=BOX_LO, hi=BOX_HI, seed=3)
+    rays = torch.from_numpy(rays_np).cuda()
+    hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+
+    # algorithmic bytes per ray from the instrumented traversal
+    cnt = scene.query_counters(rays)
+    per_ray = {k: cnt[k] / cnt["rays"] for k in ("nodes", "refs", "tris")}
+    bytes_per_ray = RAY_B + HIT_B + NODE_B * per_ray["nodes"] + REF_B * per_ray["refs"] + TRI_B * per_ray["tris"]
+
+    steps, warm = (3, 1) if args.profile else (args.steps, args.warmup)
+    for _ in range(warm):
+        scene.intersect(rays, hits)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        scene.intersect(rays, hits)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms_step = e0.elapsed_time(e1) / steps
+    mrays = n / ms_step / 1e3
+    hit_frac = float((hits[:, 0] >= 0).float().mean().item())
+
+    achieved = bytes_per_ray * n / (ms_step * 1e-3) / 1e9
+    out = {
+        "metric": "Mrays/s closest-hit (incoherent, 1.3M-tri displaced icosphere in Cornell box)",
+        "value": round(mrays, 1),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": round(ms_step, 4),
+        "wall_ms_per_step": round(wall * 1e3 / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generators; no dataset)",
+        "config": {"workload": "cornell_displaced_ico8_incoherent_closest_hit", "triangles": data.ntri,
+                   "rays": n, "refs": st["refs"], "wide_nodes": st["wide_nodes"], "hit_fraction": round(hit_frac, 4),
+                   "sbvh_build_s": round(st["build_seconds"], 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic() if n == 1 << 24 else None,
+                     "traffic_source": os.path.relpath(PROFILE, ROOT),
+                     "bytes_per_ray": round(bytes_per_ray, 1),
+                     "visits_per_ray": {k: round(v, 3) for k, v in per_ray.items()},
+                     "kernel": "k_trace<16,false,false>"},
+        "cpu_baseline": None,
+    }
+    if not (args.no_cpu or args.profile):
+        cb = cpu_baseline(data, rays_np, min(args.cpu_sample, n))
+        out["cpu_baseline"] = cb
+        if cb:
+            out["gpu_over_cpu"] = round(mrays / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -142,6 +142,14 @@ MCRT_API int mcrt_intersect(mcrt_scene* scene, const float* rays, size_t n, mcrt
                             int flags, void* stream);
 MCRT_API int mcrt_occluded(mcrt_scene* scene, const float* rays, size_t n, uint8_t* occluded,
                            int flags, void* stream);
+/* Instrumented query (no reference counterpart; measurement only): the
+ * same traversal as mcrt_intersect (any_hit = 0) or mcrt_occluded
+ * (any_hit = 1), writing no results, accumulating into counters[4] the
+ * totals of 4-wide node fetches (128 B each), leaf-reference box fetches
+ * (32 B), triangle fetches + Moller-Trumbore tests (32 B) and rays.
+ * Synchronous; rays follow `flags` like the queries. */
+MCRT_API int mcrt_query_counters(mcrt_scene* scene, const float* rays, size_t n, int any_hit,
+                                 uint64_t counters[4], int flags, void* stream);
 
 /* Wavefront path tracer.  film = W*H*3 floats, row y = 0 is the image
  * bottom (film.cpp:204-215); each pixel receives the mean of its samples'

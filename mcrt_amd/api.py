@@ -77,6 +77,7 @@ def lib():
         "mcrt_scene_num_triangles": (I, [P, ctypes.POINTER(Z)]),
         "mcrt_intersect": (I, [P, P, Z, P, I, P]),
         "mcrt_occluded": (I, [P, P, Z, P, I, P]),
+        "mcrt_query_counters": (I, [P, P, Z, I, P, I, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -229,6 +230,15 @@ class Scene:
             raise ValueError("rays and output must both be host or both be device buffers")
         _check(lib().mcrt_occluded(self.h, rp, n, op, DEVICE_PTRS if rdev else 0, _stream_of(rdev)))
         return out
+
+    def query_counters(self, rays, any_hit=False):
+        """{nodes, refs, tris, rays} fetched by the traversal of `rays`."""
+        n = rays.shape[0]
+        rp, rdev = _ptr(rays)
+        c = np.zeros(4, np.uint64)
+        _check(lib().mcrt_query_counters(self.h, rp, n, int(any_hit), c.ctypes.data, DEVICE_PTRS if rdev else 0,
+                                         _stream_of(rdev)))
+        return dict(zip(("nodes", "refs", "tris", "rays"), (int(x) for x in c)))
 
     def close(self):
         if self.h:

@@ -300,4 +300,30 @@ MCRT_API int mcrt_occluded(mcrt_scene* s, const float* rays, size_t n, uint8_t* 
     return run_query(s, rays, n, nullptr, occluded, TRACE_ANY, flags, (hipStream_t)stream, nullptr);
 }
 
+MCRT_API int mcrt_query_counters(mcrt_scene* s, const float* rays, size_t n, int any_hit, uint64_t counters[4],
+                                 int flags, void* stream)
+{
+    if (!s || !counters || (n && !rays)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_query_counters: NULL argument");
+    if (!s->finalized) return fail(MCRT_ERR_STATE, "mcrt_query_counters: scene not finalized");
+    HIPCHK(hipSetDevice(s->ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf d_cnt, d_rays, d_out;
+    int rc;
+    if ((rc = d_cnt.ensure(4 * sizeof(uint64_t)))) return rc;
+    HIPCHK(hipMemsetAsync(d_cnt.p, 0, 4 * sizeof(uint64_t), st));
+    const float* r = rays;
+    if (!(flags & MCRT_DEVICE_PTRS))
+    {
+        if ((rc = d_rays.ensure(n * 8 * sizeof(float)))) return rc;
+        HIPCHK(hipMemcpyAsync(d_rays.p, rays, n * 8 * sizeof(float), hipMemcpyHostToDevice, st));
+        r = d_rays.as<float>();
+    }
+    if ((rc = d_out.ensure(any_hit ? n : n * sizeof(mcrt_hit)))) return rc;
+    HIPCHK(launch_trace(s->dev, r, (int64_t)n, any_hit ? nullptr : d_out.p, any_hit ? d_out.as<uint8_t>() : nullptr,
+                        any_hit ? TRACE_ANY : TRACE_CLOSEST, d_cnt.as<uint64_t>(), st));
+    HIPCHK(hipMemcpyAsync(counters, d_cnt.p, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MCRT_OK;
+}
+
 }  // extern "C"
