@@ -78,6 +78,7 @@ def lib():
         "mcrt_intersect": (I, [P, P, Z, P, I, P]),
         "mcrt_occluded": (I, [P, P, Z, P, I, P]),
         "mcrt_query_counters": (I, [P, P, Z, I, P, I, P]),
+        "mcrt_render": (I, [P, ctypes.POINTER(RenderParams), P, I, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -230,6 +231,25 @@ class Scene:
             raise ValueError("rays and output must both be host or both be device buffers")
         _check(lib().mcrt_occluded(self.h, rp, n, op, DEVICE_PTRS if rdev else 0, _stream_of(rdev)))
         return out
+
+    # -- rendering ---------------------------------------------------------
+    @staticmethod
+    def params(spp=16, max_depth=8, rr_start=3, rr_prob=0.8, seed=1, sample_offset=0, nee=True):
+        return RenderParams(int(spp), int(max_depth), int(rr_start), float(rr_prob), int(seed), int(sample_offset),
+                            1 if nee else 0, 0)
+
+    def render(self, film=None, **kw):
+        """Path-traced film (H, W, 3) float32, row 0 = image bottom.  `film`
+        may be a CUDA tensor (asynchronous on the current stream)."""
+        if self.camera is None:
+            raise McrtError(2, "set_camera() first")
+        w, h = self.camera[2], self.camera[3]
+        p = self.params(**kw)
+        if film is None:
+            film = np.empty((h, w, 3), np.float32)
+        fp, dev = _ptr(film)
+        _check(lib().mcrt_render(self.h, ctypes.byref(p), fp, DEVICE_PTRS if dev else 0, _stream_of(dev)))
+        return film
 
     def query_counters(self, rays, any_hit=False):
         """{nodes, refs, tris, rays} fetched by the traversal of `rays`."""

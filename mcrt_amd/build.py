@@ -20,7 +20,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("MCRT_ARCH", "gfx950")
 
 HOST_SRCS = ["capi.cpp", "scene.cpp", "sbvh.cpp", "wide.cpp"]
-HIP_SRCS = ["traverse.hip"]
+HIP_SRCS = ["traverse.hip", "pt.hip"]
 
 INC = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-I" + os.path.join(ROCM, "include")]
 HOST_FLAGS = ["-std=c++17", "-O3", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fvisibility=hidden",

@@ -300,6 +300,28 @@ MCRT_API int mcrt_occluded(mcrt_scene* s, const float* rays, size_t n, uint8_t* 
     return run_query(s, rays, n, nullptr, occluded, TRACE_ANY, flags, (hipStream_t)stream, nullptr);
 }
 
+MCRT_API int mcrt_render(mcrt_scene* s, const mcrt_render_params* p, float* film, int flags, void* stream)
+{
+    if (!s || !p || !film) return fail(MCRT_ERR_INVALID_ARG, "mcrt_render: NULL argument");
+    if (!s->finalized) return fail(MCRT_ERR_STATE, "mcrt_render: scene not finalized");
+    if (!s->has_cam) return fail(MCRT_ERR_STATE, "mcrt_render: no camera (mcrt_scene_set_camera)");
+    if (p->spp < 1) return fail(MCRT_ERR_INVALID_ARG, "mcrt_render: spp must be >= 1");
+    if (p->max_depth < 0 || p->max_depth > 250) return fail(MCRT_ERR_INVALID_ARG, "mcrt_render: max_depth in [0, 250]");
+    if (!(p->rr_prob > 0.0f && p->rr_prob <= 1.0f)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_render: rr_prob in (0, 1]");
+    if (p->sample_offset < 0) return fail(MCRT_ERR_INVALID_ARG, "mcrt_render: sample_offset must be >= 0");
+    HIPCHK(hipSetDevice(s->ctx->device));
+    std::lock_guard<std::mutex> g(s->pt_mu);
+    hipStream_t st = (hipStream_t)stream;
+    if (flags & MCRT_DEVICE_PTRS) return render_scene(s, p, film, st);
+    const size_t bytes = (size_t)s->dev_cam.w * s->dev_cam.h * 3 * sizeof(float);
+    int rc = s->ptw.film.ensure(bytes);
+    if (rc) return rc;
+    if ((rc = render_scene(s, p, s->ptw.film.as<float>(), st))) return rc;
+    HIPCHK(hipMemcpyAsync(film, s->ptw.film.p, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MCRT_OK;
+}
+
 MCRT_API int mcrt_query_counters(mcrt_scene* s, const float* rays, size_t n, int any_hit, uint64_t counters[4],
                                  int flags, void* stream)
 {

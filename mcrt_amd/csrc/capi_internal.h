@@ -63,6 +63,15 @@ struct ObjMesh
 int load_obj(const char* path, const float* m16, ObjMesh& out);
 }  // namespace mcrt
 
+namespace mcrt
+{
+// Path-tracer workspace (SoA path state), grown on demand.
+struct PTWork
+{
+    DevBuf T, L, flags, rays[2], act[2], rec[2], cnt, hits, samples, film;
+};
+}  // namespace mcrt
+
 struct mcrt_scene
 {
     mcrt_ctx* ctx = nullptr;
@@ -84,6 +93,11 @@ struct mcrt_scene
     // staging for host-pointer queries
     std::mutex q_mu;
     mcrt::DevBuf q_in, q_out;
+    // path tracer
+    std::mutex pt_mu;
+    mcrt::PTWork ptw;
+    int64_t pt_capacity = 0;
+    int64_t pt_chunk_paths = int64_t(1) << 22;
 };
 
 namespace mcrt
@@ -93,4 +107,7 @@ int finalize_scene(mcrt_scene* s, int threads);
 int upload_materials(mcrt_scene* s);
 int run_query(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, uint8_t* occ, int mode, int flags,
               hipStream_t stream, uint64_t* counters);
+int pt_workspace(mcrt_scene* s, int64_t P);
+// Renders into a device film (W*H*3 floats) on `stream` (asynchronous).
+int render_scene(mcrt_scene* s, const mcrt_render_params* p, float* film, hipStream_t stream);
 }  // namespace mcrt

@@ -1,0 +1,403 @@
+// Wavefront path tracer (mcrt_render): the reference's rayTrace integrator
+// (demo_pt.cpp:512-655) and render loop (demo_pt.cpp:657-720) as a
+// sequence of kernels over SoA path state.
+//
+// Per chunk of P = pixels x spp paths (path p = pixel-major, sample-minor):
+//   generate             camera rays with the tent filter
+//   for k = 0 .. max_depth + 1:
+//     extend             one traversal launch over the compacted
+//                        continuation rays AND the next-event rays of
+//                        vertex k-1 (segment counts read on the device)
+//     resolve            next-event contributions of vertex k-1 (MIS),
+//                        added before anything of vertex k, as in the
+//                        reference's accumulation order
+//     shade (k <= max)   emitter hit, next-event rays at diffuse
+//                        vertices, BSDF continuation, Russian roulette
+//   finish               NaN -> 0, clamp >= 0 per sample
+//   accumulate           per-pixel sum in sample order x (1/spp)
+// Every kernel is grid-strided over a fixed grid and reads its work count
+// from device memory, so the whole chunk is launch-only (graph-capturable).
+#include <algorithm>
+
+#include "capi_internal.h"
+#include "pt_core.h"
+#include "trace_kernel.h"
+
+namespace mcrt
+{
+namespace
+{
+constexpr int kLds = 16;
+constexpr int kTPB = 256;
+
+struct PTState
+{
+    float4* T;           // throughput (xyz)
+    float4* L;           // radiance (xyz)
+    uint8_t* last_diff;  // previous vertex was diffuse
+    float4* rays[2];     // [P continuation | 2P next-event] x 2 float4 per ray
+    uint32_t* act[2];    // path id per continuation ray
+    float4* rec[2];      // next-event records, 4 float4 each
+    uint32_t* cnt[2];    // {continuation rays, next-event rays}
+    int4* hits;          // 3P
+    float4* samples;     // P clamped sample radiance
+};
+
+struct PTArgs
+{
+    int64_t P;           // paths in this chunk
+    int64_t pix0;        // first pixel of the chunk
+    int32_t spp;
+    int32_t max_depth, rr_start, nee;
+    float rr_prob;
+    uint64_t seed;
+    int64_t sample_offset;
+    DevCamera cam;
+};
+
+__device__ __forceinline__ void store_ray(float4* rays, int64_t i, V3f o, V3f d, float tmin, float tmax)
+{
+    rays[2 * i] = make_float4(o.x, o.y, o.z, tmin);
+    rays[2 * i + 1] = make_float4(d.x, d.y, d.z, tmax);
+}
+
+__global__ void k_set_counts(uint32_t* cnt, uint32_t a, uint32_t b)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+    {
+        cnt[0] = a;
+        cnt[1] = b;
+    }
+}
+
+// Camera::makeRay (camera.cpp:37-43) + tent filter (demo_pt.cpp:687-699).
+__global__ void k_generate(PTState S, PTArgs a)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t pixel = a.pix0 + p / a.spp;
+        const uint64_t sample = (uint64_t)(a.sample_offset + p % a.spp);
+        const int x = (int)(pixel % a.cam.w), y = (int)(pixel / a.cam.w);
+        float dx = 0.0f, dy = 0.0f;
+        if (a.spp > 1)
+        {
+            const uint64_t key = rng_key(a.seed, (uint64_t)pixel, sample, 255u);
+            float r1 = 2.0f * rng_u01(key, 0);
+            float r2 = 2.0f * rng_u01(key, 1);
+            dx = r1 < 1 ? sqrtf(r1) - 1.0f : 1.0f - sqrtf(2.0f - r1);
+            dy = r2 < 1 ? sqrtf(r2) - 1.0f : 1.0f - sqrtf(2.0f - r2);
+        }
+        float mx = (((float)x + 0.5f) + dx) / (float)a.cam.w;
+        float my = (((float)y + 0.5f) + dy) / (float)a.cam.h;
+        V3f su = (ld3(a.cam.fx) * (mx - 0.5f)) * a.cam.ww;
+        V3f sv = (ld3(a.cam.fy) * (my - 0.5f)) * 1.0f;
+        V3f sw = ld3(a.cam.fwd) * a.cam.dist;
+        V3f d = nrm((su + sv) + sw);
+        store_ray(S.rays[0], p, ld3(a.cam.o), d, kEpsRay, __builtin_inff());
+        S.act[0][p] = (uint32_t)p;
+        S.T[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        S.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        S.last_diff[p] = 0;
+    }
+}
+
+__device__ __forceinline__ V3f f3(float4 v) { return V3f{v.x, v.y, v.z}; }
+
+// Next-event contributions of one diffuse vertex (demo_pt.cpp:591-643 with
+// Scene::lightPdfW scene.cpp:239-267 and lightRadiance :345-366).
+__global__ void k_resolve(DevScene D, PTState S, int cur, int64_t P)
+{
+    const uint32_t nrec = S.cnt[cur][1] / 2;
+    const float4* R = S.rec[cur];
+    for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < nrec; m += (int64_t)gridDim.x * blockDim.x)
+    {
+        const float4 r0 = R[4 * m], r1 = R[4 * m + 1], r2 = R[4 * m + 2], r3 = R[4 * m + 3];
+        const V3f T = f3(r0), col = f3(r1), wb = f3(r2), wo = f3(r3);
+        const float pdf_b = r0.w, factor2 = r1.w;
+        const uint32_t path = bits_of(r2.w);
+        const int li = (int)bits_of(r3.w);
+        const int4 h1 = S.hits[P + 2 * m];
+        V3f local = v3f(0.0f, 0.0f, 0.0f);
+        float pdf_l = 0.0f;
+        V3f le1 = v3f(0.0f, 0.0f, 0.0f);
+        if (h1.x >= 0)
+        {
+            const TriShade ts = D.shade[h1.x];
+            const DevMaterial mt = D.mats[ts.mat];
+            if (mt.type == MCRT_LGHT)
+            {
+                const V3f ns1 = shading_normal(ts, float_of((uint32_t)h1.z), float_of((uint32_t)h1.w));
+                const float cosl = dot(ns1, -wb);
+                if (!(cosl < 1e-6f))
+                {
+                    const DevLight Lt = D.lights[ts.light];
+                    const float pdfA = (1.0f / Lt.area) * Lt.pdf;
+                    const float t1 = float_of((uint32_t)h1.y);
+                    pdf_l = pdfA * (t1 * t1) / cosl;
+                }
+                if (!(dot(ns1, -wb) < 0)) le1 = ld3(mt.rgb);
+            }
+        }
+        const float mis_b = pdf_b / (pdf_b + pdf_l);
+        local = local + (col * le1) * mis_b;
+        if (li >= 0)
+        {
+            const int4 h2 = S.hits[P + 2 * m + 1];
+            V3f le2 = v3f(0.0f, 0.0f, 0.0f);
+            if (h2.x >= 0)
+            {
+                const TriShade ts = D.shade[h2.x];
+                const DevMaterial mt = D.mats[ts.mat];
+                if (mt.type == MCRT_LGHT)
+                {
+                    const V3f ns2 = shading_normal(ts, float_of((uint32_t)h2.z), float_of((uint32_t)h2.w));
+                    if (!(dot(ns2, -wo) < 0) && ts.light == li) le2 = ld3(mt.rgb);
+                }
+            }
+            const float ip = 1.0f / kPi;
+            const V3f f = v3f(ip, ip, ip) * col;  // LambertianBSDF::evaluate (bsdf.cpp:96-116)
+            local = local + (f * le2) * factor2;
+        }
+        float4 L = S.L[path];
+        V3f Lv = f3(L) + T * local;
+        S.L[path] = make_float4(Lv.x, Lv.y, Lv.z, 0.0f);
+    }
+}
+
+// One vertex of every live path (demo_pt.cpp:523-651).
+__global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
+{
+    const int nxt = cur ^ 1;
+    const uint32_t A = S.cnt[cur][0];
+    const float4* rays = S.rays[cur];
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < A; j += (int64_t)gridDim.x * blockDim.x)
+    {
+        const uint32_t p = S.act[cur][j];
+        const int4 h = S.hits[j];
+        if (h.x < 0) continue;  // miss: no environment map
+        const float4 ra = rays[2 * j], rb = rays[2 * j + 1];
+        const V3f o = f3(ra), d = f3(rb);
+        const float t = float_of((uint32_t)h.y), u = float_of((uint32_t)h.z), v = float_of((uint32_t)h.w);
+        const TriShade ts = D.shade[h.x];
+        const DevMaterial mt = D.mats[ts.mat];
+        const V3f col = ld3(mt.rgb);
+        const V3f phit = o + d * t;
+        const V3f ns = shading_normal(ts, u, v);
+        const bool no_direct = !a.nee || k == 0 || !S.last_diff[p];
+        V3f T = f3(S.T[p]);
+        if (mt.type == MCRT_LGHT)
+        {
+            if (no_direct && dot(d, ns) < 0)
+            {
+                float4 L = S.L[p];
+                V3f Lv = f3(L) + T * col;
+                S.L[p] = make_float4(Lv.x, Lv.y, Lv.z, 0.0f);
+            }
+            continue;
+        }
+        S.last_diff[p] = mt.type == MCRT_DIFF;
+        const int64_t pixel = a.pix0 + p / a.spp;
+        const uint64_t sample = (uint64_t)(a.sample_offset + p % a.spp);
+        const uint64_t key = rng_key(a.seed, (uint64_t)pixel, sample, (uint32_t)k);
+        if (a.nee && mt.type == MCRT_DIFF && D.nlights > 0)
+        {
+            const V3f wb = lambert_dir(ns, rng_u01(key, 0), rng_u01(key, 1));
+            const float pdf_b = fmax_ref(dot(wb, ns), kEpsCos) / kPi;
+            // Scene::sampleLightArea (scene.cpp:298-343)
+            const int li = sample_light(D.lights, D.nlights, rng_u01(key, 2));
+            const DevLight Lt = D.lights[li];
+            const float* tv = D.verts + 9 * (int64_t)Lt.tri;
+            const float r0 = sqrtf(rng_u01(key, 3));
+            const float lu = 1.0f - r0;
+            const float lv = rng_u01(key, 4) * r0;
+            const V3f pl = (ld3(tv) * ((1.0f - lu) - lv) + ld3(tv + 3) * lu) + ld3(tv + 6) * lv;
+            const V3f nl = shading_normal(D.shade[Lt.tri], lu, lv);
+            const float pdfA = (1.0f / Lt.area) * Lt.pdf;
+            V3f wo = pl - phit;
+            const float dist2 = len2(wo);
+            wo = nrm(wo);
+            const float cosL = dot(nl, -wo);
+            float pdfW = pdfA * dist2 / fabsf(cosL);  // __pdfAtoW (geometry.cpp:222-225)
+            if (cosL <= 0) pdfW = 0;
+            float factor2 = 0.0f;
+            int li_rec = -1;
+            if (pdfW > 0)
+            {
+                const float pdf_bp = fmax_ref(dot(wo, ns), kEpsCos) / kPi;
+                const float mis_l = pdfW / (pdfW + pdf_bp);
+                const float cosr = stdmax(dot(ns, wo), 0.0f);
+                factor2 = cosr / pdfW * mis_l;
+                li_rec = li;
+            }
+            const uint32_t slot = atomicAdd(&S.cnt[nxt][1], 2u);
+            const int64_t m = slot / 2;
+            float4* R = S.rec[nxt];
+            R[4 * m] = make_float4(T.x, T.y, T.z, pdf_b);
+            R[4 * m + 1] = make_float4(col.x, col.y, col.z, factor2);
+            R[4 * m + 2] = make_float4(wb.x, wb.y, wb.z, float_of(p));
+            R[4 * m + 3] = make_float4(wo.x, wo.y, wo.z, float_of((uint32_t)li_rec));
+            float4* nr = S.rays[nxt];
+            store_ray(nr, a.P + 2 * m, phit, wb, kEpsRay, __builtin_inff());
+            if (li_rec >= 0)
+                store_ray(nr, a.P + 2 * m + 1, phit, wo, kEpsRay, __builtin_inff());
+            else  // zero direction: the slab test rejects every box (ray.cpp:77-79)
+                store_ray(nr, a.P + 2 * m + 1, phit, v3f(0.0f, 0.0f, 0.0f), kEpsRay, __builtin_inff());
+        }
+        // Continuation (Material::sample, demo_pt.cpp:645-650)
+        const V3f wi = -d;
+        V3f nd, weight;
+        if (mt.type == MCRT_GLSY)
+        {
+            float w;
+            nd = glossy_dir(wi, ns, mt.gloss, rng_u01(key, 5), rng_u01(key, 6), w);
+            weight = col * w;
+        }
+        else
+        {
+            nd = lambert_dir(ns, rng_u01(key, 5), rng_u01(key, 6));
+            weight = col;
+        }
+        T = T * weight;
+        if (k >= a.max_depth) continue;
+        if (k >= a.rr_start)
+        {
+            if (rng_u01(key, 7) >= a.rr_prob) continue;
+            T = T * (1.0f / a.rr_prob);
+        }
+        S.T[p] = make_float4(T.x, T.y, T.z, 0.0f);
+        const uint32_t slot = atomicAdd(&S.cnt[nxt][0], 1u);
+        store_ray(S.rays[nxt], slot, phit, nd, kEpsRay, __builtin_inff());
+        S.act[nxt][slot] = p;
+    }
+}
+
+// demo_pt.cpp:653 (NaN -> 0) and :703 (clamp >= 0 per channel).
+__global__ void k_finish(PTState S, int64_t P)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (int64_t)gridDim.x * blockDim.x)
+    {
+        float4 L = S.L[p];
+        if (!(L.x == L.x && L.y == L.y && L.z == L.z)) L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        S.samples[p] = make_float4(fmax_ref(0.0f, L.x), fmax_ref(0.0f, L.y), fmax_ref(0.0f, L.z), 0.0f);
+    }
+}
+
+// FrameBuffer::accumulatePixel over the samples in order, then
+// scale((float)(1.0/spp)) (demo_pt.cpp:700-710, film.cpp:204-260).
+__global__ void k_accumulate(PTState S, float* film, int64_t pix0, int64_t npix, int spp, float scale)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < npix; q += (int64_t)gridDim.x * blockDim.x)
+    {
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        for (int s = 0; s < spp; s++)
+        {
+            const float4 v = S.samples[q * spp + s];
+            gx = gx + v.x;
+            gy = gy + v.y;
+            gz = gz + v.z;
+        }
+        float* f = film + 3 * (pix0 + q);
+        f[0] = gx * scale;
+        f[1] = gy * scale;
+        f[2] = gz * scale;
+    }
+}
+
+unsigned grid_for(int64_t n, int num_cus)
+{
+    int64_t b = (n + kTPB - 1) / kTPB;
+    int64_t cap = (int64_t)num_cus * 8;
+    if (b > cap) b = cap;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+}  // namespace
+
+// Workspace for up to P paths.
+int pt_workspace(mcrt_scene* s, int64_t P)
+{
+    if (s->pt_capacity >= P) return MCRT_OK;
+    PTWork& w = s->ptw;
+    int rc;
+    const size_t n = (size_t)P;
+    if ((rc = w.T.ensure(n * 16))) return rc;
+    if ((rc = w.L.ensure(n * 16))) return rc;
+    if ((rc = w.flags.ensure(n))) return rc;
+    for (int i = 0; i < 2; i++)
+    {
+        if ((rc = w.rays[i].ensure(3 * n * 32))) return rc;
+        if ((rc = w.act[i].ensure(n * 4))) return rc;
+        if ((rc = w.rec[i].ensure(n * 64))) return rc;
+    }
+    if ((rc = w.cnt.ensure(4 * sizeof(uint32_t)))) return rc;
+    if ((rc = w.hits.ensure(3 * n * 16))) return rc;
+    if ((rc = w.samples.ensure(n * 16))) return rc;
+    s->pt_capacity = P;
+    return MCRT_OK;
+}
+
+int render_scene(mcrt_scene* s, const mcrt_render_params* prm, float* film, hipStream_t st)
+{
+    const DevCamera& cam = s->dev_cam;
+    const int64_t npix = (int64_t)cam.w * cam.h;
+    const int spp = prm->spp;
+    int64_t pix_per_chunk = s->pt_chunk_paths / spp;
+    if (pix_per_chunk < 1) pix_per_chunk = 1;
+    if (pix_per_chunk > npix) pix_per_chunk = npix;
+    int rc = pt_workspace(s, pix_per_chunk * spp);
+    if (rc) return rc;
+    PTWork& w = s->ptw;
+    PTState S;
+    S.T = w.T.as<float4>();
+    S.L = w.L.as<float4>();
+    S.last_diff = w.flags.as<uint8_t>();
+    for (int i = 0; i < 2; i++)
+    {
+        S.rays[i] = w.rays[i].as<float4>();
+        S.act[i] = w.act[i].as<uint32_t>();
+        S.rec[i] = w.rec[i].as<float4>();
+        S.cnt[i] = w.cnt.as<uint32_t>() + 2 * i;
+    }
+    S.hits = w.hits.as<int4>();
+    S.samples = w.samples.as<float4>();
+    const int cus = s->ctx->num_cus;
+    const float scale = (float)(1.0 / spp);
+    for (int64_t pix0 = 0; pix0 < npix; pix0 += pix_per_chunk)
+    {
+        const int64_t np = std::min(pix_per_chunk, npix - pix0);
+        PTArgs a;
+        a.P = np * spp;
+        a.pix0 = pix0;
+        a.spp = spp;
+        a.max_depth = prm->max_depth;
+        a.rr_start = prm->rr_start;
+        a.nee = prm->nee;
+        a.rr_prob = prm->rr_prob;
+        a.seed = prm->seed;
+        a.sample_offset = prm->sample_offset;
+        a.cam = cam;
+        const unsigned g = grid_for(a.P, cus);
+        hipLaunchKernelGGL(k_generate, dim3(g), dim3(kTPB), 0, st, S, a);
+        hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[0], (uint32_t)a.P, 0u);
+        int cur = 0;
+        for (int k = 0; k <= prm->max_depth + 1; k++)
+        {
+            const int nxt = cur ^ 1;
+            hipLaunchKernelGGL((k_trace<kLds, false, false, true>), dim3((unsigned)trace_blocks(s->dev, 3 * a.P)),
+                               dim3(kWave), 0, st, s->dev, S.rays[cur], (int64_t)0, S.cnt[cur], a.P, S.hits,
+                               (uint8_t*)nullptr, (unsigned long long*)nullptr);
+            hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kTPB), 0, st, s->dev, S, cur, a.P);
+            if (k <= prm->max_depth)
+            {
+                hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[nxt], 0u, 0u);
+                hipLaunchKernelGGL(k_shade, dim3(g), dim3(kTPB), 0, st, s->dev, S, a, cur, k);
+            }
+            cur = nxt;
+        }
+        hipLaunchKernelGGL(k_finish, dim3(g), dim3(kTPB), 0, st, S, a.P);
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(np, cus)), dim3(kTPB), 0, st, S, film, pix0, np, spp, scale);
+        HIPCHK(hipGetLastError());
+    }
+    return MCRT_OK;
+}
+}  // namespace mcrt
