@@ -71,6 +71,55 @@ def cpu_baseline(scene_data, rays_np, sample):
             "hit_fraction": round(float((prim >= 0).mean()), 4)}
 
 
+RENDER = dict(width=1024, height=1024, spp=64, max_depth=8, rr_start=3, rr_prob=0.8, seed=1)
+
+
+def render_leg(scene, data, steps, warm, cpu):
+    """Msamples/s of the full 1024^2 x 64 spp render (depth 8, roulette
+    from vertex 3), and the oracle's fp32 integrator on a band of rows."""
+    import torch
+    from mcrt_amd import scenes
+    R = RENDER
+    cam = scenes.CAMERA
+    scene.set_camera(cam["origin"], cam["target"], R["width"], R["height"], cam["fov_y"])
+    kw = {k: R[k] for k in ("spp", "max_depth", "rr_start", "rr_prob", "seed")}
+    film = torch.zeros((R["height"], R["width"], 3), dtype=torch.float32, device="cuda")
+    for _ in range(warm):
+        scene.render(film, **kw)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(torch.cuda.current_stream())
+    for _ in range(steps):
+        scene.render(film, **kw)
+    e1.record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nsamp = R["width"] * R["height"] * R["spp"]
+    out = {"msamples_per_s": round(nsamp / ms / 1e3, 2), "ms_per_frame": round(ms, 2),
+           "config": dict(R, scene="cornell_displaced_ico8", nee=True), "film_mean": round(float(film.mean()), 6)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _libs
+        osc = _libs.OracleScene(data)
+        cores = _libs.oracle().orc_num_procs()
+        rows = (R["height"] // 2 - 8, R["height"] // 2 + 8)
+        t0 = time.time()
+        cfilm = osc.render(R["width"], R["height"], R["spp"], R["max_depth"], R["rr_start"], R["rr_prob"],
+                           R["seed"], rows=rows, threads=cores)
+        dt = time.time() - t0
+        cs = (rows[1] - rows[0]) * R["width"] * R["spp"]
+        band = film[rows[0]:rows[1]].cpu().numpy()
+        same = bool(np.array_equal(band.view(np.uint32), cfilm[rows[0]:rows[1]].view(np.uint32)))
+        out["cpu_baseline"] = {"value": round(cs / dt / 1e6, 4), "unit": "Msamples/s", "cores": int(cores),
+                               "kind": "port",
+                               "sample": f"rows {rows[0]}..{rows[1]} of the same frame ({cs} samples), oracle fp32 "
+                                         "integrator, OpenMP over pixels",
+                               "band_bit_identical_to_gpu": same}
+        out["gpu_over_cpu"] = round(out["msamples_per_s"] / out["cpu_baseline"]["value"], 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +127,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rays", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-render", action="store_true", help="skip the path-tracing leg")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -171,6 +221,12 @@ def main():
         out["cpu_baseline"] = cb
         if cb:
             out["gpu_over_cpu"] = round(mrays / cb["value"], 1)
+    if not args.no_render:
+        del rays, hits
+        torch.cuda.empty_cache()
+        r = render_leg(scene, data, 1 if args.profile else 2, 1, cpu=not (args.no_cpu or args.profile))
+        out["render"] = r
+        out["msamples_per_s"] = r["msamples_per_s"]
     print(json.dumps(out), flush=True)
 
 

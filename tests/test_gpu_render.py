@@ -9,21 +9,8 @@ import _libs
 pytestmark = pytest.mark.gpu
 
 
-def oracle_render(d, w, h, spp, max_depth, rr_start, rr_prob, seed, nee, precision=0, camera=None, samples=False):
-    from mcrt_amd import scenes
-    L = _libs.oracle()
-    cam = camera or scenes.CAMERA
-    hs = L.orc_scene_create(_libs.ptr(d.tris), None if d.normals is None else _libs.ptr(d.normals),
-                            _libs.ptr(d.tri_mat), d.ntri, _libs.ptr(d.mat_type), _libs.ptr(d.mat_rgb),
-                            _libs.ptr(d.mat_gloss), len(d.mat_type))
-    ip = np.array([w, h, spp, max_depth, rr_start, 1 if nee else 0, precision], np.int32)
-    fp = np.array([rr_prob, *cam["origin"], *cam["target"], cam["fov_y"]], np.float32)
-    film = np.zeros((h, w, 3), np.float32)
-    smp = np.zeros((h * w * spp, 3), np.float32) if samples else None
-    L.orc_render(hs, _libs.ptr(ip), _libs.ptr(fp), seed, 0, _libs.ptr(film), None if smp is None else _libs.ptr(smp),
-                 8)
-    L.orc_scene_free(hs)
-    return (film, smp) if samples else film
+def oracle_render(d, w, h, spp, max_depth, rr_start, rr_prob, seed, nee, precision=0):
+    return _libs.OracleScene(d).render(w, h, spp, max_depth, rr_start, rr_prob, seed, nee=nee, precision=precision)
 
 
 def gpu_render(ctx, d, w, h, **kw):
