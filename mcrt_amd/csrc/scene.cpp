@@ -14,7 +14,11 @@
 namespace mcrt
 {
 // Mesh::Mesh (mesh.cpp:24-60): whitespace tokens; `v x y z`, `f a b c`
-// (1-based), everything else skipped token by token.  The optional
+// (1-based), everything else skipped token by token.  A record is kept only
+// when all of its numbers were extracted: at end of file the reference's
+// failed `>> token` repeats the previous token and, after a face line, adds
+// a phantom face with indices -1 that it then reads and writes out of
+// bounds; every real face is loaded identically.  The optional
 // column-major transform is applied per vertex in GLM's order
 // (m0*x + m1*y) + (m2*z + m3*w) (type_mat4x4.inl:532-543), then
 // Mesh::genNormals (mesh.cpp:66-92): vertex normal = normalize of the sum
@@ -32,16 +36,17 @@ int load_obj(const char* path, const float* m16, ObjMesh& m)
         if (token == "v")
         {
             float a, b, c;
-            reader >> a >> b >> c;
-            m.v.push_back(f3(a, b, c));
+            if (reader >> a >> b >> c) m.v.push_back(f3(a, b, c));
         }
         else if (token == "f")
         {
             unsigned int a, b, c;
-            reader >> a >> b >> c;
-            m.f.push_back((int)(a - 1));
-            m.f.push_back((int)(b - 1));
-            m.f.push_back((int)(c - 1));
+            if (reader >> a >> b >> c)
+            {
+                m.f.push_back((int)(a - 1));
+                m.f.push_back((int)(b - 1));
+                m.f.push_back((int)(c - 1));
+            }
         }
     }
     for (int idx : m.f)

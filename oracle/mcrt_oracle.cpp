@@ -628,6 +628,11 @@ static void digest_rec(const BVH& b, int ni, Fnv& h)
 
 // ---------------------------------------------------------------- OBJ ----
 // Mesh::Mesh (mesh.cpp:24-60): whitespace tokens, `v x y z` and `f a b c`.
+// At end of file the reference's last `reader >> token` fails and leaves the
+// previous token in place; if that was "f" it appends a phantom face with
+// indices -1 (extraction failures store 0) and later reads and writes
+// m_verts[-1] / m_vertex_normals[-1].  Only completely extracted records are
+// kept here, which is the reference's result for every real face.
 struct Mesh
 {
     std::vector<V3> v;
@@ -648,14 +653,15 @@ static bool load_obj(const char* path, Mesh& m)
         if (token == "v")
         {
             float a, b, c;
-            reader >> a >> b >> c;
-            m.v.push_back(v3(a, b, c));
+            if (reader >> a >> b >> c) m.v.push_back(v3(a, b, c));
         }
         else if (token == "f")
         {
             unsigned int a, b, c;
-            reader >> a >> b >> c;
-            m.f.push_back((int)(a - 1)); m.f.push_back((int)(b - 1)); m.f.push_back((int)(c - 1));
+            if (reader >> a >> b >> c)
+            {
+                m.f.push_back((int)(a - 1)); m.f.push_back((int)(b - 1)); m.f.push_back((int)(c - 1));
+            }
         }
     }
     return true;

@@ -219,10 +219,22 @@ REF_API void ref_clip_bound_tight(const float* tri9, const float* box6, float lo
 
 // Mesh + genNormals (mesh.cpp:24-118) and the loadModel vertex transform
 // (scene.cpp:374-381) with the given column-major matrix (NULL = identity).
+// Faces with an index outside the vertex array (the phantom face Mesh
+// appends at end of file) are not returned.
+static bool face_ok(const Mesh& m, size_t i)
+{
+    const long long nv = (long long)m.m_verts.size();
+    const int idx[3] = {m.m_faces[i].x, m.m_faces[i].y, m.m_faces[i].z};
+    for (int k = 0; k < 3; k++)
+        if (idx[k] < 0 || idx[k] >= nv) return false;
+    return true;
+}
+
 REF_API long long ref_load_obj(const char* path, const float* m16, float* tris9, float* vn9)
 {
     Mesh obj(path);
-    long long nf = (long long)obj.m_faces.size();
+    long long nf = 0;
+    for (size_t i = 0; i < obj.m_faces.size(); i++) nf += face_ok(obj, i) ? 1 : 0;
     if (!tris9) return nf;
     if (m16)
     {
@@ -237,9 +249,12 @@ REF_API long long ref_load_obj(const char* path, const float* m16, float* tris9,
         }
         obj.genNormals();
     }
-    for (long long i = 0; i < nf; i++)
+    long long i = -1;
+    for (size_t f = 0; f < obj.m_faces.size(); f++)
     {
-        int idx[3] = {obj.m_faces[i].x, obj.m_faces[i].y, obj.m_faces[i].z};
+        if (!face_ok(obj, f)) continue;
+        i++;
+        int idx[3] = {obj.m_faces[f].x, obj.m_faces[f].y, obj.m_faces[f].z};
         for (int k = 0; k < 3; k++)
         {
             tris9[9 * i + 3 * k] = obj.m_verts[idx[k]].x;

@@ -49,6 +49,8 @@ def oracle():
         "orc_rng_u01": (F, [U64, U64, U64, ctypes.c_uint32, ctypes.c_uint32]),
         "orc_own_sincos": (None, [P, LL, P, P]),
         "orc_own_pow": (None, [P, P, LL, P]),
+        "orc_bsdf_sample": (None, [I, F, I, P, LL, P]),
+        "orc_light_tables": (None, [P, I, P, LL, P, P, P]),
     })
 
 
@@ -198,3 +200,4 @@ def soup_slice(n_total=1 << 22, seed=3, lo=0.0, hi=0.25):
     c = t.reshape(-1, 3, 3).mean(axis=1)
     keep = np.all((c >= lo) & (c < hi), axis=1)
     return np.ascontiguousarray(t[keep])
+

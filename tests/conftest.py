@@ -65,3 +65,7 @@ def ctx():
     import mcrt_amd
     c = mcrt_amd.Context(0)
     yield c
+
+GOLDEN = os.path.join(TESTS, "golden")
+if GOLDEN not in sys.path:
+    sys.path.insert(0, GOLDEN)

@@ -1,0 +1,169 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference
+(oracle/_ref/libmcrt_ref.so, built from the reference's own sources by
+oracle/Makefile).  Inputs are regenerated deterministically by the tests
+(seeded numpy); each fixture stores a SHA-256 of its inputs so a drift of
+the generators is detected instead of silently comparing different data.
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import _libs  # noqa: E402
+from _libs import ptr  # noqa: E402
+
+NRAYS = 16384
+
+
+def sha(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def ref_bvh(R, tris):
+    tris = np.ascontiguousarray(tris, np.float32)
+    h = R.ref_bvh_build(ptr(tris), tris.shape[0])
+    st = np.zeros(5, np.int64)
+    d = R.ref_bvh_digest(h, ptr(st))
+    return h, d, st
+
+
+def ref_hits(R, h, rays):
+    n = rays.shape[0]
+    prim = np.empty(n, np.int32)
+    tuv = np.empty((n, 3), np.float32)
+    R.ref_bvh_intersect(h, ptr(rays), n, ptr(prim), ptr(tuv))
+    return prim, tuv
+
+
+def scene_inputs():
+    """name -> (tris, rays) as the tests regenerate them."""
+    from mcrt_amd import scenes
+    dm = _libs.debug_mesh()
+    ci = scenes.cornell_icosphere(4).tris
+    ss = _libs.soup_slice()
+    return {
+        "debug_mesh": (dm, _libs.random_rays(dm, NRAYS, seed=101)),
+        "cornell_ico4": (ci, scenes.incoherent_rays(NRAYS, lo=np.array([1.0, 1.0, 1.0]),
+                                                    hi=np.array([54.0, 53.0, 54.0]), seed=102)),
+        "soup_slice": (ss, _libs.random_rays(ss, NRAYS, seed=103, margin=0.0)),
+    }
+
+
+def primitive_inputs(seed=7, n=20000):
+    rng = np.random.default_rng(seed)
+    lo = rng.uniform(-1, 1, (n, 3))
+    box = np.concatenate([lo, lo + rng.uniform(0, 1, (n, 3))], 1).astype(np.float32)
+    o = rng.uniform(-2, 2, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    zero = rng.uniform(size=n) < 0.05  # zero direction components
+    d[zero, rng.integers(0, 3, zero.sum())] = 0.0
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = o
+    rays[:, 3] = 1e-4
+    rays[:, 4:7] = d
+    rays[:, 7] = np.where(rng.uniform(size=n) < 0.3, rng.uniform(0.1, 3, n), np.inf)
+    tris = rng.uniform(-1, 1, (n, 9)).astype(np.float32)
+    tiny = rng.uniform(size=n) < 0.1  # near the absolute determinant cut
+    tris[tiny, 3:] = tris[tiny, :3].repeat(2, 0).reshape(-1, 6) + rng.uniform(-1e-3, 1e-3, (tiny.sum(), 6))
+    slab = np.sort(rng.uniform(-1, 1, (n, 2)), 1).astype(np.float32)
+    axis = rng.integers(0, 3, n).astype(np.int32)
+    return box, rays, tris, slab, axis
+
+
+def main():
+    R = _libs.reference()
+    if R is None:
+        sys.exit("oracle/_ref/libmcrt_ref.so is missing: make -C oracle ref")
+    # DebugMesh triangles (the reference's own test mesh, debug_mesh.cpp:16-759)
+    n = R.ref_debug_mesh(None)
+    dm = np.zeros((n, 9), np.float32)
+    R.ref_debug_mesh(ptr(dm))
+    np.savez_compressed(os.path.join(HERE, "debug_mesh.npz"), tris=dm)
+
+    for name, (tris, rays) in scene_inputs().items():
+        h, d, st = ref_bvh(R, tris)
+        prim, tuv = ref_hits(R, h, rays)
+        R.ref_bvh_free(h)
+        np.savez_compressed(os.path.join(HERE, f"bvh_{name}.npz"), digest=np.uint64(d), stats=st, prim=prim,
+                            tuv=tuv, input_sha=sha(tris, rays))
+        print(name, hex(d), st, "hit", (prim >= 0).mean())
+
+    box, rays, tris, slab, axis = primitive_inputs()
+    m = box.shape[0]
+    ov = np.array([R.ref_ray_box(ptr(rays[i]), ptr(box[i])) for i in range(m)], np.int32)
+    tuv = np.zeros((m, 3), np.float32)
+    hit = np.zeros(m, np.int32)
+    for i in range(m):
+        out = np.zeros(3, np.float32)
+        hit[i] = R.ref_ray_tri(ptr(rays[i]), ptr(tris[i]), ptr(out))
+        tuv[i] = out
+    clip = np.zeros((m, 6), np.float32)
+    tbox = np.concatenate([tris.reshape(-1, 3, 3).min(1), tris.reshape(-1, 3, 3).max(1)], 1).astype(np.float32)
+    for i in range(m):
+        R.ref_clip_bound_tight(ptr(tris[i]), ptr(tbox[i]), float(slab[i, 0]), float(slab[i, 1]), int(axis[i]),
+                               ptr(clip[i]))
+    np.savez_compressed(os.path.join(HERE, "primitives.npz"), overlap=ov, mt_hit=hit, mt_tuv=tuv, clip=clip,
+                        input_sha=sha(box, rays, tris, slab, axis))
+    print("primitives", ov.mean(), hit.mean())
+
+    # Camera::makeRay at pixel centres and corners of a 64x48 film
+    from mcrt_amd import scenes
+    cam = scenes.CAMERA
+    o = np.array(cam["origin"], np.float32)
+    t = np.array(cam["target"], np.float32)
+    xs, ys = np.meshgrid(np.arange(0, 65, 4, dtype=np.float32), np.arange(0, 49, 4, dtype=np.float32))
+    pxy = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.float32)
+    cr = np.zeros((pxy.shape[0], 6), np.float32)
+    R.ref_camera_rays(ptr(o), ptr(t), 64, 48, cam["fov_y"], ptr(pxy), pxy.shape[0], ptr(cr))
+    np.savez_compressed(os.path.join(HERE, "camera.npz"), pxy=pxy, rays=cr)
+
+    # BSDF sample / pdf / evaluate (material.cpp:102-127)
+    rng = np.random.default_rng(11)
+    nb = 4096
+    nrm = rng.normal(size=(nb, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    wi = rng.normal(size=(nb, 3))
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    wi = np.where((wi * nrm).sum(1, keepdims=True) < 0, -wi, wi)
+    inp = np.concatenate([wi, nrm, rng.uniform(0, 1, (nb, 2))], 1).astype(np.float32)
+    rgb = np.array([0.7, 0.5, 0.3], np.float32)
+    outs = {}
+    for typ in (1, 4):
+        out = np.zeros((nb, 10), np.float32)
+        R.ref_bsdf(typ, ptr(rgb), ptr(inp), nb, ptr(out))
+        outs[f"type{typ}"] = out
+    np.savez_compressed(os.path.join(HERE, "bsdf.npz"), inputs=inp, rgb=rgb, **outs)
+
+    # DiscreteSampler (sample.cpp:11-118) over light areas
+    areas = np.array([3.0, 0.5, 7.25, 1e-3, 2.0, 2.0], np.float32)
+    r = np.concatenate([np.linspace(0, 1, 1001, dtype=np.float32), rng.uniform(0, 1, 1000).astype(np.float32)])
+    idx = np.zeros(r.shape[0], np.int32)
+    pdf = np.zeros(areas.shape[0], np.float32)
+    R.ref_discrete(ptr(areas), areas.shape[0], ptr(r), r.shape[0], ptr(idx), ptr(pdf))
+    np.savez_compressed(os.path.join(HERE, "discrete.npz"), areas=areas, r=r, idx=idx, pdf=pdf)
+
+    # OBJ ingest (Mesh + genNormals) of a generated file
+    path = os.path.join(HERE, "cornell_ico2.obj")
+    scenes.write_obj(path, scenes.cornell_icosphere(2).tris)
+    nf = R.ref_load_obj(path.encode(), None, None, None)
+    ot = np.zeros((nf, 9), np.float32)
+    on = np.zeros((nf, 9), np.float32)
+    R.ref_load_obj(path.encode(), None, ptr(ot), ptr(on))
+    np.savez_compressed(os.path.join(HERE, "obj_cornell_ico2.npz"), tris=ot, normals=on)
+    print("obj faces", nf)
+
+
+if __name__ == "__main__":
+    main()
