@@ -70,6 +70,74 @@ def test_render_device_film_and_chunking(ctx):
     _compare("device film", film.cpu().numpy(), o)
 
 
+def test_render_matches_fp64_high_spp(ctx):
+    """GPU film at 1024 spp vs an independent FP64 oracle render at 8192 spp.
+
+    sigma per pixel = standard error of the GPU estimator, from the
+    per-sample values of the fp32 oracle (bit-identical to the GPU).  The
+    per-sample distributions are strongly right-skewed (light paths), so the
+    per-pixel 3-sigma bound is required for >= 98.5% of pixel channels rather
+    than the Gaussian 99.7%; bias is tested on the image mean, where the skew
+    averages out; the image relative RMSE must be noise-limited."""
+    from mcrt_amd import scenes
+    d = scenes.cornell_icosphere(3)
+    w = h = 24
+    spp, spp_ref = 1024, 8192
+    kw = dict(max_depth=6, rr_start=3, rr_prob=0.8, nee=True)
+    g = gpu_render(ctx, d, w, h, spp=spp, seed=21, **kw)
+    osc = _libs.OracleScene(d)
+    _, smp = osc.render(w, h, spp, seed=21, samples=True, threads=16, **kw)
+    var = smp.reshape(h, w, spp, 3).var(axis=2, ddof=1)
+    sig2 = var / spp + var / spp_ref
+    ref = osc.render(w, h, spp_ref, seed=99, precision=1, threads=16, **kw)
+    within = np.abs(g - ref) <= 3.0 * np.sqrt(sig2) + 1e-7
+    assert within.mean() >= 0.985, within.mean()
+    bias_z = (g - ref).mean() / np.sqrt(sig2.sum() / sig2.size ** 2)
+    assert abs(bias_z) < 4.0, bias_z
+    rel = np.sqrt(np.mean((g - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+    expected = np.sqrt(np.mean(sig2)) / np.sqrt(np.mean(ref ** 2))
+    # sample variances of skewed data underestimate the true variance
+    assert rel < 1.5 * expected and rel < 0.08, (rel, expected)
+
+
+def test_obj_scene_matches_oracle(ctx, tmp_path):
+    """Scene::loadModel path: OBJ -> product loader -> queries and render
+    equal the oracle's on the oracle-loaded triangles and normals."""
+    import mcrt_amd
+    from mcrt_amd import scenes
+    d = scenes.cornell_icosphere(2)
+    geo, light = str(tmp_path / "geo.obj"), str(tmp_path / "light.obj")
+    scenes.write_obj(light, d.tris[:2])
+    scenes.write_obj(geo, d.tris[2:])
+    s = mcrt_amd.Scene(ctx)
+    ml = s.add_material(mcrt_amd.LGHT, (15.0, 15.0, 15.0))
+    md = s.add_material(mcrt_amd.DIFF, (0.6, 0.6, 0.6))
+    assert s.load_model(light, ml) == 2
+    assert s.load_model(geo, md) == d.ntri - 2
+    s.set_camera(scenes.CAMERA["origin"], scenes.CAMERA["target"], 24, 24, scenes.CAMERA["fov_y"])
+    s.finalize()
+    L = _libs.oracle()
+    parts = []
+    for p in (light, geo):
+        n = L.orc_load_obj(p.encode(), None, None)
+        t = np.zeros((n, 9), np.float32)
+        vn = np.zeros((n, 9), np.float32)
+        L.orc_load_obj(p.encode(), _libs.ptr(t), _libs.ptr(vn))
+        parts.append((t, vn))
+    t = np.concatenate([parts[0][0], parts[1][0]])
+    vn = np.concatenate([parts[0][1], parts[1][1]])
+    rays = scenes.incoherent_rays(8192, lo=np.array([1.0, 1.0, 1.0]), hi=np.array([54.0, 53.0, 54.0]), seed=4)
+    prim, _ = mcrt_amd.split_hits(s.intersect(rays))
+    op, _ = _libs.OracleBVH(t).intersect(rays)
+    assert np.array_equal(prim, op)
+    od = scenes.SceneData(t, vn, np.r_[[0, 0], np.ones(t.shape[0] - 2, np.int32)], [0, 1],
+                          [(15.0,) * 3, (0.6,) * 3], [25.0, 25.0])
+    kw = dict(spp=4, max_depth=4, rr_start=2, rr_prob=0.8, seed=5, nee=True)
+    g = s.render(**kw)
+    o = _libs.OracleScene(od).render(24, 24, **kw)
+    _compare("obj scene", g, o)
+
+
 def test_render_errors(ctx):
     import mcrt_amd
     from mcrt_amd import scenes
