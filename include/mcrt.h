@@ -157,6 +157,21 @@ MCRT_API int mcrt_query_counters(mcrt_scene* scene, const float* rays, size_t n,
 MCRT_API int mcrt_render(mcrt_scene* scene, const mcrt_render_params* params, float* film,
                          int flags, void* stream);
 
+/* Path-replay adjoint of mcrt_render (no reference counterpart; the
+ * differentiable-rendering extension of the integrator).  Given dLoss/dfilm
+ * (W*H*3) for the film that mcrt_render(params) produces, writes the
+ * gradients with respect to the colour of each triangle's material:
+ * d_albedo[3*t+c] for diffuse / glossy triangles and d_emission[3*t+c] for
+ * light triangles (ntri*3 floats each, overwritten; the other array holds 0
+ * for that triangle).  A material shared by several triangles has the sum
+ * of their entries as its gradient.  The same paths are replayed (the RNG is
+ * counter-based), so memory does not grow with path length.  `film`
+ * (optional, W*H*3) receives the primal film.  Gradients are not formed for
+ * colour channels that are exactly 0. */
+MCRT_API int mcrt_render_adjoint(mcrt_scene* scene, const mcrt_render_params* params,
+                                 const float* dL_dfilm, float* d_albedo, float* d_emission,
+                                 float* film, int flags, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

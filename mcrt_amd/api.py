@@ -79,6 +79,7 @@ def lib():
         "mcrt_occluded": (I, [P, P, Z, P, I, P]),
         "mcrt_query_counters": (I, [P, P, Z, I, P, I, P]),
         "mcrt_render": (I, [P, ctypes.POINTER(RenderParams), P, I, P]),
+        "mcrt_render_adjoint": (I, [P, ctypes.POINTER(RenderParams), P, P, P, P, I, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -250,6 +251,26 @@ class Scene:
         fp, dev = _ptr(film)
         _check(lib().mcrt_render(self.h, ctypes.byref(p), fp, DEVICE_PTRS if dev else 0, _stream_of(dev)))
         return film
+
+    def render_adjoint(self, dfilm, d_albedo=None, d_emission=None, film=None, **kw):
+        """Path-replay gradients of <dfilm, film> with respect to every
+        triangle's material colour: (d_albedo, d_emission), (ntri, 3) each.
+        All buffers host (numpy) or all device (torch, current stream)."""
+        n = self.num_triangles()
+        if d_albedo is None:
+            d_albedo = np.empty((n, 3), np.float32)
+        if d_emission is None:
+            d_emission = np.empty((n, 3), np.float32)
+        p = self.params(**kw)
+        dp, dev = _ptr(dfilm)
+        ap, adev = _ptr(d_albedo)
+        ep, edev = _ptr(d_emission)
+        fp, fdev = _ptr(film) if film is not None else (None, dev)
+        if not (dev == adev == edev == fdev):
+            raise ValueError("all buffers must be host or all device")
+        _check(lib().mcrt_render_adjoint(self.h, ctypes.byref(p), dp, ap, ep, fp, DEVICE_PTRS if dev else 0,
+                                         _stream_of(dev)))
+        return d_albedo, d_emission
 
     def query_counters(self, rays, any_hit=False):
         """{nodes, refs, tris, rays} fetched by the traversal of `rays`."""

@@ -300,24 +300,60 @@ MCRT_API int mcrt_occluded(mcrt_scene* s, const float* rays, size_t n, uint8_t* 
     return run_query(s, rays, n, nullptr, occluded, TRACE_ANY, flags, (hipStream_t)stream, nullptr);
 }
 
+static int check_render(const mcrt_scene* s, const mcrt_render_params* p, const char* who)
+{
+    if (!s->finalized) return fail(MCRT_ERR_STATE, "%s: scene not finalized", who);
+    if (!s->has_cam) return fail(MCRT_ERR_STATE, "%s: no camera (mcrt_scene_set_camera)", who);
+    if (p->spp < 1) return fail(MCRT_ERR_INVALID_ARG, "%s: spp must be >= 1", who);
+    if (p->max_depth < 0 || p->max_depth > 250) return fail(MCRT_ERR_INVALID_ARG, "%s: max_depth in [0, 250]", who);
+    if (!(p->rr_prob > 0.0f && p->rr_prob <= 1.0f)) return fail(MCRT_ERR_INVALID_ARG, "%s: rr_prob in (0, 1]", who);
+    if (p->sample_offset < 0) return fail(MCRT_ERR_INVALID_ARG, "%s: sample_offset must be >= 0", who);
+    return MCRT_OK;
+}
+
 MCRT_API int mcrt_render(mcrt_scene* s, const mcrt_render_params* p, float* film, int flags, void* stream)
 {
     if (!s || !p || !film) return fail(MCRT_ERR_INVALID_ARG, "mcrt_render: NULL argument");
-    if (!s->finalized) return fail(MCRT_ERR_STATE, "mcrt_render: scene not finalized");
-    if (!s->has_cam) return fail(MCRT_ERR_STATE, "mcrt_render: no camera (mcrt_scene_set_camera)");
-    if (p->spp < 1) return fail(MCRT_ERR_INVALID_ARG, "mcrt_render: spp must be >= 1");
-    if (p->max_depth < 0 || p->max_depth > 250) return fail(MCRT_ERR_INVALID_ARG, "mcrt_render: max_depth in [0, 250]");
-    if (!(p->rr_prob > 0.0f && p->rr_prob <= 1.0f)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_render: rr_prob in (0, 1]");
-    if (p->sample_offset < 0) return fail(MCRT_ERR_INVALID_ARG, "mcrt_render: sample_offset must be >= 0");
+    int rc = check_render(s, p, "mcrt_render");
+    if (rc) return rc;
     HIPCHK(hipSetDevice(s->ctx->device));
     std::lock_guard<std::mutex> g(s->pt_mu);
     hipStream_t st = (hipStream_t)stream;
-    if (flags & MCRT_DEVICE_PTRS) return render_scene(s, p, film, st);
+    if (flags & MCRT_DEVICE_PTRS) return render_scene(s, p, film, nullptr, st);
     const size_t bytes = (size_t)s->dev_cam.w * s->dev_cam.h * 3 * sizeof(float);
-    int rc = s->ptw.film.ensure(bytes);
-    if (rc) return rc;
-    if ((rc = render_scene(s, p, s->ptw.film.as<float>(), st))) return rc;
+    if ((rc = s->ptw.film.ensure(bytes))) return rc;
+    if ((rc = render_scene(s, p, s->ptw.film.as<float>(), nullptr, st))) return rc;
     HIPCHK(hipMemcpyAsync(film, s->ptw.film.p, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_render_adjoint(mcrt_scene* s, const mcrt_render_params* p, const float* dL_dfilm,
+                                 float* d_albedo, float* d_emission, float* film, int flags, void* stream)
+{
+    if (!s || !p || !dL_dfilm || !d_albedo || !d_emission)
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_render_adjoint: NULL argument");
+    int rc = check_render(s, p, "mcrt_render_adjoint");
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(s->ctx->device));
+    std::lock_guard<std::mutex> g(s->pt_mu);
+    hipStream_t st = (hipStream_t)stream;
+    if (flags & MCRT_DEVICE_PTRS)
+    {
+        AdjointIO io{dL_dfilm, d_albedo, d_emission};
+        return render_scene(s, p, film, &io, st);
+    }
+    PTWork& w = s->ptw;
+    const size_t fb = (size_t)s->dev_cam.w * s->dev_cam.h * 3 * sizeof(float);
+    const size_t gb = (size_t)s->dev.ntri * 3 * sizeof(float);
+    if ((rc = w.dfilm.ensure(fb)) || (rc = w.galb.ensure(gb)) || (rc = w.gem.ensure(gb))) return rc;
+    if (film && (rc = w.film.ensure(fb))) return rc;
+    HIPCHK(hipMemcpyAsync(w.dfilm.p, dL_dfilm, fb, hipMemcpyHostToDevice, st));
+    AdjointIO io{w.dfilm.as<float>(), w.galb.as<float>(), w.gem.as<float>()};
+    if ((rc = render_scene(s, p, film ? w.film.as<float>() : nullptr, &io, st))) return rc;
+    HIPCHK(hipMemcpyAsync(d_albedo, w.galb.p, gb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(d_emission, w.gem.p, gb, hipMemcpyDeviceToHost, st));
+    if (film) HIPCHK(hipMemcpyAsync(film, w.film.p, fb, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return MCRT_OK;
 }

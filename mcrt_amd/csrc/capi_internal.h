@@ -69,6 +69,16 @@ namespace mcrt
 struct PTWork
 {
     DevBuf T, L, flags, rays[2], act[2], rec[2], cnt, hits, samples, film;
+    DevBuf Ltot, delta;                    // adjoint
+    DevBuf dfilm, galb, gem;               // adjoint staging for host pointers
+};
+
+// Device buffers of one adjoint render.
+struct AdjointIO
+{
+    const float* dfilm;  // W*H*3 dLoss/dfilm
+    float* d_albedo;     // ntri*3
+    float* d_emission;   // ntri*3
 };
 }  // namespace mcrt
 
@@ -107,7 +117,9 @@ int finalize_scene(mcrt_scene* s, int threads);
 int upload_materials(mcrt_scene* s);
 int run_query(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, uint8_t* occ, int mode, int flags,
               hipStream_t stream, uint64_t* counters);
-int pt_workspace(mcrt_scene* s, int64_t P);
-// Renders into a device film (W*H*3 floats) on `stream` (asynchronous).
-int render_scene(mcrt_scene* s, const mcrt_render_params* p, float* film, hipStream_t stream);
+int pt_workspace(mcrt_scene* s, int64_t P, bool adjoint);
+// Renders into a device film (W*H*3 floats, may be NULL with adj) on
+// `stream` (asynchronous); with adj also the path-replay gradients.
+int render_scene(mcrt_scene* s, const mcrt_render_params* p, float* film, const AdjointIO* adj,
+                 hipStream_t stream);
 }  // namespace mcrt

@@ -1,4 +1,5 @@
-// Wavefront path tracer (mcrt_render): the reference's rayTrace integrator
+// Wavefront path tracer (mcrt_render) and its path-replay adjoint
+// (mcrt_render_adjoint): the reference's rayTrace integrator
 // (demo_pt.cpp:512-655) and render loop (demo_pt.cpp:657-720) as a
 // sequence of kernels over SoA path state.
 //
@@ -17,6 +18,17 @@
 //   accumulate           per-pixel sum in sample order x (1/spp)
 // Every kernel is grid-strided over a fixed grid and reads its work count
 // from device memory, so the whole chunk is launch-only (graph-capturable).
+//
+// Adjoint (path replay, Vicini, Speierer, Jakob, "Path Replay
+// Backpropagation", SIGGRAPH 2021): sampling, MIS weights and the roulette
+// (fixed survival probability) do not depend on albedo or emission, so
+// every term of a path's radiance is a product of the albedos of the
+// vertices before it and one emission.  Pass 1 runs the chunk forward and
+// keeps each path's radiance L; with delta = dLoss/dfilm * [L >= 0] / spp,
+// pass 2 replays the same paths (same counter-based random numbers) and
+// adds delta * (L - L_before_vertex) / albedo to the vertex's triangle and
+// delta * (coefficient of Le) to the emitting triangle of every emission
+// term.  Memory is independent of path length.
 #include <algorithm>
 
 #include "capi_internal.h"
@@ -41,6 +53,10 @@ struct PTState
     uint32_t* cnt[2];    // {continuation rays, next-event rays}
     int4* hits;          // 3P
     float4* samples;     // P clamped sample radiance
+    float4* Ltot;        // adjoint: radiance of pass 1
+    float4* delta;       // adjoint: dLoss/dL per path
+    float* g_alb;        // adjoint: ntri x 3
+    float* g_em;         // adjoint: ntri x 3
 };
 
 struct PTArgs
@@ -60,6 +76,46 @@ __device__ __forceinline__ void store_ray(float4* rays, int64_t i, V3f o, V3f d,
     rays[2 * i] = make_float4(o.x, o.y, o.z, tmin);
     rays[2 * i + 1] = make_float4(d.x, d.y, d.z, tmax);
 }
+__device__ __forceinline__ V3f f3(float4 v) { return V3f{v.x, v.y, v.z}; }
+__device__ __forceinline__ float4 f4(V3f v) { return make_float4(v.x, v.y, v.z, 0.0f); }
+
+// Wave-reduced scatter-add of a 3-vector into g[3*key]: lanes holding the
+// same key are summed across the wavefront first (up to 4 distinct keys per
+// call -- the lights and the large walls most lanes share), the rest go
+// straight to the hardware float atomics.  Every lane of the wave must call
+// it (the loops below iterate uniformly per wave).
+__device__ __forceinline__ void scatter3(float* g, int key, V3f v, bool valid)
+{
+    uint64_t pending = __ballot(valid);
+    for (int it = 0; it < 4 && pending; ++it)
+    {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int k = __shfl(key, leader);
+        const bool mine = valid && key == k;
+        const uint64_t same = __ballot(mine);
+        float sx = mine ? v.x : 0.0f, sy = mine ? v.y : 0.0f, sz = mine ? v.z : 0.0f;
+        for (int off = 32; off > 0; off >>= 1)
+        {
+            sx += __shfl_xor(sx, off);
+            sy += __shfl_xor(sy, off);
+            sz += __shfl_xor(sz, off);
+        }
+        if ((int)(threadIdx.x & 63) == leader)
+        {
+            unsafeAtomicAdd(g + 3 * (int64_t)k, sx);
+            unsafeAtomicAdd(g + 3 * (int64_t)k + 1, sy);
+            unsafeAtomicAdd(g + 3 * (int64_t)k + 2, sz);
+        }
+        valid = valid && !mine;
+        pending &= ~same;
+    }
+    if (valid)
+    {
+        unsafeAtomicAdd(g + 3 * (int64_t)key, v.x);
+        unsafeAtomicAdd(g + 3 * (int64_t)key + 1, v.y);
+        unsafeAtomicAdd(g + 3 * (int64_t)key + 2, v.z);
+    }
+}
 
 __global__ void k_set_counts(uint32_t* cnt, uint32_t a, uint32_t b)
 {
@@ -68,6 +124,12 @@ __global__ void k_set_counts(uint32_t* cnt, uint32_t a, uint32_t b)
         cnt[0] = a;
         cnt[1] = b;
     }
+}
+
+__global__ void k_fill(float* p, int64_t n, float v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 // Camera::makeRay (camera.cpp:37-43) + tent filter (demo_pt.cpp:687-699).
@@ -101,173 +163,225 @@ __global__ void k_generate(PTState S, PTArgs a)
     }
 }
 
-__device__ __forceinline__ V3f f3(float4 v) { return V3f{v.x, v.y, v.z}; }
-
 // Next-event contributions of one diffuse vertex (demo_pt.cpp:591-643 with
 // Scene::lightPdfW scene.cpp:239-267 and lightRadiance :345-366).
+template <bool ADJ>
 __global__ void k_resolve(DevScene D, PTState S, int cur, int64_t P)
 {
     const uint32_t nrec = S.cnt[cur][1] / 2;
     const float4* R = S.rec[cur];
-    for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < nrec; m += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nrec; base += (int64_t)gridDim.x * blockDim.x)
     {
-        const float4 r0 = R[4 * m], r1 = R[4 * m + 1], r2 = R[4 * m + 2], r3 = R[4 * m + 3];
-        const V3f T = f3(r0), col = f3(r1), wb = f3(r2), wo = f3(r3);
-        const float pdf_b = r0.w, factor2 = r1.w;
-        const uint32_t path = bits_of(r2.w);
-        const int li = (int)bits_of(r3.w);
-        const int4 h1 = S.hits[P + 2 * m];
-        V3f local = v3f(0.0f, 0.0f, 0.0f);
-        float pdf_l = 0.0f;
-        V3f le1 = v3f(0.0f, 0.0f, 0.0f);
-        if (h1.x >= 0)
+        const int64_t m = base + threadIdx.x;
+        const bool act = m < nrec;
+        int em1 = -1, em2 = -1;
+        V3f g1 = v3f(0.0f, 0.0f, 0.0f), g2 = g1;
+        if (act)
         {
-            const TriShade ts = D.shade[h1.x];
-            const DevMaterial mt = D.mats[ts.mat];
-            if (mt.type == MCRT_LGHT)
+            const float4 r0 = R[4 * m], r1 = R[4 * m + 1], r2 = R[4 * m + 2], r3 = R[4 * m + 3];
+            const V3f T = f3(r0), col = f3(r1), wb = f3(r2), wo = f3(r3);
+            const float pdf_b = r0.w, factor2 = r1.w;
+            const uint32_t path = bits_of(r2.w);
+            const int li = (int)bits_of(r3.w);
+            const int4 h1 = S.hits[P + 2 * m];
+            V3f local = v3f(0.0f, 0.0f, 0.0f);
+            float pdf_l = 0.0f;
+            V3f le1 = v3f(0.0f, 0.0f, 0.0f);
+            if (h1.x >= 0)
             {
-                const V3f ns1 = shading_normal(ts, float_of((uint32_t)h1.z), float_of((uint32_t)h1.w));
-                const float cosl = dot(ns1, -wb);
-                if (!(cosl < 1e-6f))
-                {
-                    const DevLight Lt = D.lights[ts.light];
-                    const float pdfA = (1.0f / Lt.area) * Lt.pdf;
-                    const float t1 = float_of((uint32_t)h1.y);
-                    pdf_l = pdfA * (t1 * t1) / cosl;
-                }
-                if (!(dot(ns1, -wb) < 0)) le1 = ld3(mt.rgb);
-            }
-        }
-        const float mis_b = pdf_b / (pdf_b + pdf_l);
-        local = local + (col * le1) * mis_b;
-        if (li >= 0)
-        {
-            const int4 h2 = S.hits[P + 2 * m + 1];
-            V3f le2 = v3f(0.0f, 0.0f, 0.0f);
-            if (h2.x >= 0)
-            {
-                const TriShade ts = D.shade[h2.x];
+                const TriShade ts = D.shade[h1.x];
                 const DevMaterial mt = D.mats[ts.mat];
                 if (mt.type == MCRT_LGHT)
                 {
-                    const V3f ns2 = shading_normal(ts, float_of((uint32_t)h2.z), float_of((uint32_t)h2.w));
-                    if (!(dot(ns2, -wo) < 0) && ts.light == li) le2 = ld3(mt.rgb);
+                    const V3f ns1 = shading_normal(ts, float_of((uint32_t)h1.z), float_of((uint32_t)h1.w));
+                    const float cosl = dot(ns1, -wb);
+                    if (!(cosl < 1e-6f))
+                    {
+                        const DevLight Lt = D.lights[ts.light];
+                        const float pdfA = (1.0f / Lt.area) * Lt.pdf;
+                        const float t1 = float_of((uint32_t)h1.y);
+                        pdf_l = pdfA * (t1 * t1) / cosl;
+                    }
+                    if (!(dot(ns1, -wb) < 0))
+                    {
+                        le1 = ld3(mt.rgb);
+                        em1 = h1.x;
+                    }
                 }
             }
-            const float ip = 1.0f / kPi;
-            const V3f f = v3f(ip, ip, ip) * col;  // LambertianBSDF::evaluate (bsdf.cpp:96-116)
-            local = local + (f * le2) * factor2;
+            const float mis_b = pdf_b / (pdf_b + pdf_l);
+            local = local + (col * le1) * mis_b;
+            V3f f = v3f(0.0f, 0.0f, 0.0f);
+            if (li >= 0)
+            {
+                const int4 h2 = S.hits[P + 2 * m + 1];
+                V3f le2 = v3f(0.0f, 0.0f, 0.0f);
+                if (h2.x >= 0)
+                {
+                    const TriShade ts = D.shade[h2.x];
+                    const DevMaterial mt = D.mats[ts.mat];
+                    if (mt.type == MCRT_LGHT)
+                    {
+                        const V3f ns2 = shading_normal(ts, float_of((uint32_t)h2.z), float_of((uint32_t)h2.w));
+                        if (!(dot(ns2, -wo) < 0) && ts.light == li)
+                        {
+                            le2 = ld3(mt.rgb);
+                            em2 = h2.x;
+                        }
+                    }
+                }
+                const float ip = 1.0f / kPi;
+                f = v3f(ip, ip, ip) * col;  // LambertianBSDF::evaluate (bsdf.cpp:96-116)
+                local = local + (f * le2) * factor2;
+            }
+            V3f Lv = f3(S.L[path]) + T * local;
+            S.L[path] = f4(Lv);
+            if (ADJ)
+            {
+                const V3f dl = f3(S.delta[path]);
+                g1 = (dl * T) * (col * mis_b);
+                g2 = (dl * T) * (f * factor2);
+            }
         }
-        float4 L = S.L[path];
-        V3f Lv = f3(L) + T * local;
-        S.L[path] = make_float4(Lv.x, Lv.y, Lv.z, 0.0f);
+        if (ADJ)
+        {
+            scatter3(S.g_em, em1, g1, em1 >= 0);
+            scatter3(S.g_em, em2, g2, em2 >= 0);
+        }
     }
 }
 
 // One vertex of every live path (demo_pt.cpp:523-651).
+template <bool ADJ>
 __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
 {
     const int nxt = cur ^ 1;
     const uint32_t A = S.cnt[cur][0];
     const float4* rays = S.rays[cur];
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < A; j += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < A; base += (int64_t)gridDim.x * blockDim.x)
     {
-        const uint32_t p = S.act[cur][j];
-        const int4 h = S.hits[j];
-        if (h.x < 0) continue;  // miss: no environment map
-        const float4 ra = rays[2 * j], rb = rays[2 * j + 1];
-        const V3f o = f3(ra), d = f3(rb);
-        const float t = float_of((uint32_t)h.y), u = float_of((uint32_t)h.z), v = float_of((uint32_t)h.w);
-        const TriShade ts = D.shade[h.x];
-        const DevMaterial mt = D.mats[ts.mat];
-        const V3f col = ld3(mt.rgb);
-        const V3f phit = o + d * t;
-        const V3f ns = shading_normal(ts, u, v);
-        const bool no_direct = !a.nee || k == 0 || !S.last_diff[p];
-        V3f T = f3(S.T[p]);
-        if (mt.type == MCRT_LGHT)
+        const int64_t j = base + threadIdx.x;
+        int alb_key = -1, em_key = -1;
+        V3f g_alb = v3f(0.0f, 0.0f, 0.0f), g_em = g_alb;
+        do
         {
-            if (no_direct && dot(d, ns) < 0)
+            if (j >= A) break;
+            const uint32_t p = S.act[cur][j];
+            const int4 h = S.hits[j];
+            if (h.x < 0) break;  // miss: no environment map
+            const float4 ra = rays[2 * j], rb = rays[2 * j + 1];
+            const V3f o = f3(ra), d = f3(rb);
+            const float t = float_of((uint32_t)h.y), u = float_of((uint32_t)h.z), v = float_of((uint32_t)h.w);
+            const TriShade ts = D.shade[h.x];
+            const DevMaterial mt = D.mats[ts.mat];
+            const V3f col = ld3(mt.rgb);
+            const V3f phit = o + d * t;
+            const V3f ns = shading_normal(ts, u, v);
+            const bool no_direct = !a.nee || k == 0 || !S.last_diff[p];
+            V3f T = f3(S.T[p]);
+            if (mt.type == MCRT_LGHT)
             {
-                float4 L = S.L[p];
-                V3f Lv = f3(L) + T * col;
-                S.L[p] = make_float4(Lv.x, Lv.y, Lv.z, 0.0f);
+                if (no_direct && dot(d, ns) < 0)
+                {
+                    V3f Lv = f3(S.L[p]) + T * col;
+                    S.L[p] = f4(Lv);
+                    if (ADJ)
+                    {
+                        em_key = h.x;
+                        g_em = f3(S.delta[p]) * T;
+                    }
+                }
+                break;
             }
-            continue;
-        }
-        S.last_diff[p] = mt.type == MCRT_DIFF;
-        const int64_t pixel = a.pix0 + p / a.spp;
-        const uint64_t sample = (uint64_t)(a.sample_offset + p % a.spp);
-        const uint64_t key = rng_key(a.seed, (uint64_t)pixel, sample, (uint32_t)k);
-        if (a.nee && mt.type == MCRT_DIFF && D.nlights > 0)
-        {
-            const V3f wb = lambert_dir(ns, rng_u01(key, 0), rng_u01(key, 1));
-            const float pdf_b = fmax_ref(dot(wb, ns), kEpsCos) / kPi;
-            // Scene::sampleLightArea (scene.cpp:298-343)
-            const int li = sample_light(D.lights, D.nlights, rng_u01(key, 2));
-            const DevLight Lt = D.lights[li];
-            const float* tv = D.verts + 9 * (int64_t)Lt.tri;
-            const float r0 = sqrtf(rng_u01(key, 3));
-            const float lu = 1.0f - r0;
-            const float lv = rng_u01(key, 4) * r0;
-            const V3f pl = (ld3(tv) * ((1.0f - lu) - lv) + ld3(tv + 3) * lu) + ld3(tv + 6) * lv;
-            const V3f nl = shading_normal(D.shade[Lt.tri], lu, lv);
-            const float pdfA = (1.0f / Lt.area) * Lt.pdf;
-            V3f wo = pl - phit;
-            const float dist2 = len2(wo);
-            wo = nrm(wo);
-            const float cosL = dot(nl, -wo);
-            float pdfW = pdfA * dist2 / fabsf(cosL);  // __pdfAtoW (geometry.cpp:222-225)
-            if (cosL <= 0) pdfW = 0;
-            float factor2 = 0.0f;
-            int li_rec = -1;
-            if (pdfW > 0)
+            if (ADJ)
             {
-                const float pdf_bp = fmax_ref(dot(wo, ns), kEpsCos) / kPi;
-                const float mis_l = pdfW / (pdfW + pdf_bp);
-                const float cosr = stdmax(dot(ns, wo), 0.0f);
-                factor2 = cosr / pdfW * mis_l;
-                li_rec = li;
+                // every later term carries this vertex's albedo once
+                const V3f rest = f3(S.Ltot[p]) - f3(S.L[p]);
+                const V3f dl = f3(S.delta[p]);
+                alb_key = h.x;
+                g_alb = v3f(col.x > 0.0f ? dl.x * rest.x / col.x : 0.0f,
+                            col.y > 0.0f ? dl.y * rest.y / col.y : 0.0f,
+                            col.z > 0.0f ? dl.z * rest.z / col.z : 0.0f);
             }
-            const uint32_t slot = atomicAdd(&S.cnt[nxt][1], 2u);
-            const int64_t m = slot / 2;
-            float4* R = S.rec[nxt];
-            R[4 * m] = make_float4(T.x, T.y, T.z, pdf_b);
-            R[4 * m + 1] = make_float4(col.x, col.y, col.z, factor2);
-            R[4 * m + 2] = make_float4(wb.x, wb.y, wb.z, float_of(p));
-            R[4 * m + 3] = make_float4(wo.x, wo.y, wo.z, float_of((uint32_t)li_rec));
-            float4* nr = S.rays[nxt];
-            store_ray(nr, a.P + 2 * m, phit, wb, kEpsRay, __builtin_inff());
-            if (li_rec >= 0)
-                store_ray(nr, a.P + 2 * m + 1, phit, wo, kEpsRay, __builtin_inff());
-            else  // zero direction: the slab test rejects every box (ray.cpp:77-79)
-                store_ray(nr, a.P + 2 * m + 1, phit, v3f(0.0f, 0.0f, 0.0f), kEpsRay, __builtin_inff());
-        }
-        // Continuation (Material::sample, demo_pt.cpp:645-650)
-        const V3f wi = -d;
-        V3f nd, weight;
-        if (mt.type == MCRT_GLSY)
+            S.last_diff[p] = mt.type == MCRT_DIFF;
+            const int64_t pixel = a.pix0 + p / a.spp;
+            const uint64_t sample = (uint64_t)(a.sample_offset + p % a.spp);
+            const uint64_t key = rng_key(a.seed, (uint64_t)pixel, sample, (uint32_t)k);
+            if (a.nee && mt.type == MCRT_DIFF && D.nlights > 0)
+            {
+                const V3f wb = lambert_dir(ns, rng_u01(key, 0), rng_u01(key, 1));
+                const float pdf_b = fmax_ref(dot(wb, ns), kEpsCos) / kPi;
+                // Scene::sampleLightArea (scene.cpp:298-343)
+                const int li = sample_light(D.lights, D.nlights, rng_u01(key, 2));
+                const DevLight Lt = D.lights[li];
+                const float* tv = D.verts + 9 * (int64_t)Lt.tri;
+                const float r0 = sqrtf(rng_u01(key, 3));
+                const float lu = 1.0f - r0;
+                const float lv = rng_u01(key, 4) * r0;
+                const V3f pl = (ld3(tv) * ((1.0f - lu) - lv) + ld3(tv + 3) * lu) + ld3(tv + 6) * lv;
+                const V3f nl = shading_normal(D.shade[Lt.tri], lu, lv);
+                const float pdfA = (1.0f / Lt.area) * Lt.pdf;
+                V3f wo = pl - phit;
+                const float dist2 = len2(wo);
+                wo = nrm(wo);
+                const float cosL = dot(nl, -wo);
+                float pdfW = pdfA * dist2 / fabsf(cosL);  // __pdfAtoW (geometry.cpp:222-225)
+                if (cosL <= 0) pdfW = 0;
+                float factor2 = 0.0f;
+                int li_rec = -1;
+                if (pdfW > 0)
+                {
+                    const float pdf_bp = fmax_ref(dot(wo, ns), kEpsCos) / kPi;
+                    const float mis_l = pdfW / (pdfW + pdf_bp);
+                    const float cosr = stdmax(dot(ns, wo), 0.0f);
+                    factor2 = cosr / pdfW * mis_l;
+                    li_rec = li;
+                }
+                const uint32_t slot = atomicAdd(&S.cnt[nxt][1], 2u);
+                const int64_t m = slot / 2;
+                float4* R = S.rec[nxt];
+                R[4 * m] = make_float4(T.x, T.y, T.z, pdf_b);
+                R[4 * m + 1] = make_float4(col.x, col.y, col.z, factor2);
+                R[4 * m + 2] = make_float4(wb.x, wb.y, wb.z, float_of(p));
+                R[4 * m + 3] = make_float4(wo.x, wo.y, wo.z, float_of((uint32_t)li_rec));
+                float4* nr = S.rays[nxt];
+                store_ray(nr, a.P + 2 * m, phit, wb, kEpsRay, __builtin_inff());
+                if (li_rec >= 0)
+                    store_ray(nr, a.P + 2 * m + 1, phit, wo, kEpsRay, __builtin_inff());
+                else  // zero direction: the slab test rejects every box (ray.cpp:77-79)
+                    store_ray(nr, a.P + 2 * m + 1, phit, v3f(0.0f, 0.0f, 0.0f), kEpsRay, __builtin_inff());
+            }
+            // Continuation (Material::sample, demo_pt.cpp:645-650)
+            const V3f wi = -d;
+            V3f nd, weight;
+            if (mt.type == MCRT_GLSY)
+            {
+                float w;
+                nd = glossy_dir(wi, ns, mt.gloss, rng_u01(key, 5), rng_u01(key, 6), w);
+                weight = col * w;
+            }
+            else
+            {
+                nd = lambert_dir(ns, rng_u01(key, 5), rng_u01(key, 6));
+                weight = col;
+            }
+            T = T * weight;
+            if (k >= a.max_depth) break;
+            if (k >= a.rr_start)
+            {
+                if (rng_u01(key, 7) >= a.rr_prob) break;
+                T = T * (1.0f / a.rr_prob);
+            }
+            S.T[p] = f4(T);
+            const uint32_t slot = atomicAdd(&S.cnt[nxt][0], 1u);
+            store_ray(S.rays[nxt], slot, phit, nd, kEpsRay, __builtin_inff());
+            S.act[nxt][slot] = p;
+        } while (false);
+        if (ADJ)
         {
-            float w;
-            nd = glossy_dir(wi, ns, mt.gloss, rng_u01(key, 5), rng_u01(key, 6), w);
-            weight = col * w;
+            scatter3(S.g_alb, alb_key, g_alb, alb_key >= 0);
+            scatter3(S.g_em, em_key, g_em, em_key >= 0);
         }
-        else
-        {
-            nd = lambert_dir(ns, rng_u01(key, 5), rng_u01(key, 6));
-            weight = col;
-        }
-        T = T * weight;
-        if (k >= a.max_depth) continue;
-        if (k >= a.rr_start)
-        {
-            if (rng_u01(key, 7) >= a.rr_prob) continue;
-            T = T * (1.0f / a.rr_prob);
-        }
-        S.T[p] = make_float4(T.x, T.y, T.z, 0.0f);
-        const uint32_t slot = atomicAdd(&S.cnt[nxt][0], 1u);
-        store_ray(S.rays[nxt], slot, phit, nd, kEpsRay, __builtin_inff());
-        S.act[nxt][slot] = p;
     }
 }
 
@@ -279,6 +393,23 @@ __global__ void k_finish(PTState S, int64_t P)
         float4 L = S.L[p];
         if (!(L.x == L.x && L.y == L.y && L.z == L.z)) L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         S.samples[p] = make_float4(fmax_ref(0.0f, L.x), fmax_ref(0.0f, L.y), fmax_ref(0.0f, L.z), 0.0f);
+    }
+}
+
+// Adjoint seed: keep pass 1's radiance and form dLoss/dL of each path
+// through the clamp (slope 1 where L >= 0), the NaN rule and the 1/spp
+// scale of the film.
+__global__ void k_delta(PTState S, PTArgs a, const float* dfilm, float scale)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += (int64_t)gridDim.x * blockDim.x)
+    {
+        const float4 L = S.L[p];
+        S.Ltot[p] = L;
+        const bool nan = !(L.x == L.x && L.y == L.y && L.z == L.z);
+        const float* g = dfilm + 3 * (a.pix0 + p / a.spp);
+        S.delta[p] = make_float4(!nan && L.x >= 0.0f ? g[0] * scale : 0.0f,
+                                 !nan && L.y >= 0.0f ? g[1] * scale : 0.0f,
+                                 !nan && L.z >= 0.0f ? g[2] * scale : 0.0f, 0.0f);
     }
 }
 
@@ -311,32 +442,62 @@ unsigned grid_for(int64_t n, int num_cus)
     if (b < 1) b = 1;
     return (unsigned)b;
 }
+
+// The bounce loop of one chunk (generate .. last resolve).
+template <bool ADJ>
+void bounce_loop(mcrt_scene* s, PTState& S, const PTArgs& a, int max_depth, unsigned g, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_generate, dim3(g), dim3(kTPB), 0, st, S, a);
+    hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[0], (uint32_t)a.P, 0u);
+    int cur = 0;
+    for (int k = 0; k <= max_depth + 1; k++)
+    {
+        const int nxt = cur ^ 1;
+        hipLaunchKernelGGL((k_trace<kLds, false, false, true>), dim3((unsigned)trace_blocks(s->dev, 3 * a.P)),
+                           dim3(kWave), 0, st, s->dev, S.rays[cur], (int64_t)0, S.cnt[cur], a.P, S.hits,
+                           (uint8_t*)nullptr, (unsigned long long*)nullptr);
+        hipLaunchKernelGGL(k_resolve<ADJ>, dim3(g), dim3(kTPB), 0, st, s->dev, S, cur, a.P);
+        if (k <= max_depth)
+        {
+            hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[nxt], 0u, 0u);
+            hipLaunchKernelGGL(k_shade<ADJ>, dim3(g), dim3(kTPB), 0, st, s->dev, S, a, cur, k);
+        }
+        cur = nxt;
+    }
+}
 }  // namespace
 
 // Workspace for up to P paths.
-int pt_workspace(mcrt_scene* s, int64_t P)
+int pt_workspace(mcrt_scene* s, int64_t P, bool adjoint)
 {
-    if (s->pt_capacity >= P) return MCRT_OK;
     PTWork& w = s->ptw;
     int rc;
     const size_t n = (size_t)P;
-    if ((rc = w.T.ensure(n * 16))) return rc;
-    if ((rc = w.L.ensure(n * 16))) return rc;
-    if ((rc = w.flags.ensure(n))) return rc;
-    for (int i = 0; i < 2; i++)
+    if (s->pt_capacity < P)
     {
-        if ((rc = w.rays[i].ensure(3 * n * 32))) return rc;
-        if ((rc = w.act[i].ensure(n * 4))) return rc;
-        if ((rc = w.rec[i].ensure(n * 64))) return rc;
+        if ((rc = w.T.ensure(n * 16))) return rc;
+        if ((rc = w.L.ensure(n * 16))) return rc;
+        if ((rc = w.flags.ensure(n))) return rc;
+        for (int i = 0; i < 2; i++)
+        {
+            if ((rc = w.rays[i].ensure(3 * n * 32))) return rc;
+            if ((rc = w.act[i].ensure(n * 4))) return rc;
+            if ((rc = w.rec[i].ensure(n * 64))) return rc;
+        }
+        if ((rc = w.cnt.ensure(4 * sizeof(uint32_t)))) return rc;
+        if ((rc = w.hits.ensure(3 * n * 16))) return rc;
+        if ((rc = w.samples.ensure(n * 16))) return rc;
+        s->pt_capacity = P;
     }
-    if ((rc = w.cnt.ensure(4 * sizeof(uint32_t)))) return rc;
-    if ((rc = w.hits.ensure(3 * n * 16))) return rc;
-    if ((rc = w.samples.ensure(n * 16))) return rc;
-    s->pt_capacity = P;
+    if (adjoint)
+    {
+        if ((rc = w.Ltot.ensure(n * 16))) return rc;
+        if ((rc = w.delta.ensure(n * 16))) return rc;
+    }
     return MCRT_OK;
 }
 
-int render_scene(mcrt_scene* s, const mcrt_render_params* prm, float* film, hipStream_t st)
+int render_scene(mcrt_scene* s, const mcrt_render_params* prm, float* film, const AdjointIO* adj, hipStream_t st)
 {
     const DevCamera& cam = s->dev_cam;
     const int64_t npix = (int64_t)cam.w * cam.h;
@@ -344,7 +505,7 @@ int render_scene(mcrt_scene* s, const mcrt_render_params* prm, float* film, hipS
     int64_t pix_per_chunk = s->pt_chunk_paths / spp;
     if (pix_per_chunk < 1) pix_per_chunk = 1;
     if (pix_per_chunk > npix) pix_per_chunk = npix;
-    int rc = pt_workspace(s, pix_per_chunk * spp);
+    int rc = pt_workspace(s, pix_per_chunk * spp, adj != nullptr);
     if (rc) return rc;
     PTWork& w = s->ptw;
     PTState S;
@@ -360,8 +521,18 @@ int render_scene(mcrt_scene* s, const mcrt_render_params* prm, float* film, hipS
     }
     S.hits = w.hits.as<int4>();
     S.samples = w.samples.as<float4>();
+    S.Ltot = adj ? w.Ltot.as<float4>() : nullptr;
+    S.delta = adj ? w.delta.as<float4>() : nullptr;
+    S.g_alb = adj ? adj->d_albedo : nullptr;
+    S.g_em = adj ? adj->d_emission : nullptr;
     const int cus = s->ctx->num_cus;
     const float scale = (float)(1.0 / spp);
+    if (adj)
+    {
+        const int64_t ng = 3 * s->dev.ntri;
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(ng, cus)), dim3(kTPB), 0, st, adj->d_albedo, ng, 0.0f);
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(ng, cus)), dim3(kTPB), 0, st, adj->d_emission, ng, 0.0f);
+    }
     for (int64_t pix0 = 0; pix0 < npix; pix0 += pix_per_chunk)
     {
         const int64_t np = std::min(pix_per_chunk, npix - pix0);
@@ -377,25 +548,18 @@ int render_scene(mcrt_scene* s, const mcrt_render_params* prm, float* film, hipS
         a.sample_offset = prm->sample_offset;
         a.cam = cam;
         const unsigned g = grid_for(a.P, cus);
-        hipLaunchKernelGGL(k_generate, dim3(g), dim3(kTPB), 0, st, S, a);
-        hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[0], (uint32_t)a.P, 0u);
-        int cur = 0;
-        for (int k = 0; k <= prm->max_depth + 1; k++)
+        bounce_loop<false>(s, S, a, prm->max_depth, g, st);
+        if (adj)
         {
-            const int nxt = cur ^ 1;
-            hipLaunchKernelGGL((k_trace<kLds, false, false, true>), dim3((unsigned)trace_blocks(s->dev, 3 * a.P)),
-                               dim3(kWave), 0, st, s->dev, S.rays[cur], (int64_t)0, S.cnt[cur], a.P, S.hits,
-                               (uint8_t*)nullptr, (unsigned long long*)nullptr);
-            hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kTPB), 0, st, s->dev, S, cur, a.P);
-            if (k <= prm->max_depth)
-            {
-                hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[nxt], 0u, 0u);
-                hipLaunchKernelGGL(k_shade, dim3(g), dim3(kTPB), 0, st, s->dev, S, a, cur, k);
-            }
-            cur = nxt;
+            hipLaunchKernelGGL(k_delta, dim3(g), dim3(kTPB), 0, st, S, a, adj->dfilm, scale);
+            bounce_loop<true>(s, S, a, prm->max_depth, g, st);
         }
-        hipLaunchKernelGGL(k_finish, dim3(g), dim3(kTPB), 0, st, S, a.P);
-        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(np, cus)), dim3(kTPB), 0, st, S, film, pix0, np, spp, scale);
+        if (film)
+        {
+            hipLaunchKernelGGL(k_finish, dim3(g), dim3(kTPB), 0, st, S, a.P);
+            hipLaunchKernelGGL(k_accumulate, dim3(grid_for(np, cus)), dim3(kTPB), 0, st, S, film, pix0, np, spp,
+                               scale);
+        }
         HIPCHK(hipGetLastError());
     }
     return MCRT_OK;

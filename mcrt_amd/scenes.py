@@ -79,6 +79,37 @@ def icosphere(sub):
     return v, f
 
 
+def geodesic(freq):
+    """Unit geodesic sphere of frequency `freq` (every icosahedron face cut
+    into freq^2 triangles on a barycentric grid, projected to the sphere):
+    20*freq^2 faces, shared vertices, outward winding."""
+    iv, ifc = icosphere(0)
+    verts, faces = [], []
+    base = 0
+    for a, b, c in ifc:
+        A, B, C = iv[a], iv[b], iv[c]
+        idx = {}
+        pts = []
+        for i in range(freq + 1):
+            for j in range(freq + 1 - i):
+                idx[(i, j)] = base + len(pts)
+                pts.append(A + (B - A) * (i / freq) + (C - A) * (j / freq))
+        for i in range(freq):
+            for j in range(freq - i):
+                faces.append((idx[(i, j)], idx[(i + 1, j)], idx[(i, j + 1)]))
+                if i + j < freq - 1:
+                    faces.append((idx[(i + 1, j)], idx[(i + 1, j + 1)], idx[(i, j + 1)]))
+        verts.extend(pts)
+        base += len(pts)
+    v = np.array(verts)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array(faces, np.int64)
+    # merge the copies of edge vertices shared by adjacent icosahedron faces
+    key = np.round(v * 1e9).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    return v[first], inv.reshape(-1)[f]
+
+
 def value_noise3(p, octaves=3, seed=7, lattice=32, base_freq=2.0):
     """3-octave value noise in [-1, 1] at points p (N,3) in [-1,1]^3.
 
@@ -179,10 +210,12 @@ def _face_normals(t):
 
 
 def cornell_displaced(sub=8, radius=SPHERE_RADIUS, amp=0.05, noise_seed=7, albedo_seed=11, phong_frac=0.1,
-                      phong_exp=64.0, emission=15.0, albedo_range=(0.2, 0.9), per_tri_albedo=True):
-    """The at-scale configuration: displaced icosphere with per-triangle materials."""
+                      phong_exp=64.0, emission=15.0, albedo_range=(0.2, 0.9), per_tri_albedo=True, geodesic_freq=None,
+                      rgb_albedo=False):
+    """The at-scale configuration: displaced icosphere with per-triangle
+    materials (or a geodesic sphere of frequency geodesic_freq)."""
     box, kinds = cornell_box()
-    v, f = icosphere(sub)
+    v, f = geodesic(geodesic_freq) if geodesic_freq else icosphere(sub)
     disp = 1.0 + amp * value_noise3(v, seed=noise_seed)
     sv = BOX_CENTER + radius * v * disp[:, None]
     vn = vertex_normals(sv, f)
@@ -190,7 +223,7 @@ def cornell_displaced(sub=8, radius=SPHERE_RADIUS, amp=0.05, noise_seed=7, albed
     sph_n = vn[f].astype(np.float32)
     ns = len(f)
     rng = np.random.default_rng(albedo_seed)
-    alb = rng.uniform(albedo_range[0], albedo_range[1], ns)
+    alb = rng.uniform(albedo_range[0], albedo_range[1], (ns, 3) if rgb_albedo else ns)
     phong = rng.uniform(0.0, 1.0, ns) < phong_frac
     grey, red, green = _box_materials()
     kind_mat = {"light": 0, "red": 2, "green": 3, "floor": 1, "back": 1, "ceiling": 1}
@@ -199,7 +232,7 @@ def cornell_displaced(sub=8, radius=SPHERE_RADIUS, amp=0.05, noise_seed=7, albed
         mat_type = np.concatenate([[api.LGHT, api.DIFF, api.DIFF, api.DIFF],
                                    np.where(phong, api.GLSY, api.DIFF)]).astype(np.int32)
         mat_rgb = np.concatenate([np.array([(emission,) * 3, grey, red, green]),
-                                  np.repeat(alb[:, None], 3, axis=1)])
+                                  alb if rgb_albedo else np.repeat(alb[:, None], 3, axis=1)])
         sph_mat = 4 + np.arange(ns, dtype=np.int32)
     else:
         mat_type = np.array([api.LGHT, api.DIFF, api.DIFF, api.DIFF, api.DIFF], np.int32)
@@ -211,6 +244,20 @@ def cornell_displaced(sub=8, radius=SPHERE_RADIUS, amp=0.05, noise_seed=7, albed
     normals = np.concatenate([box_n, sph_n])
     tri_mat = np.concatenate([box_mat, sph_mat])
     return SceneData(tris, normals, tri_mat, mat_type, mat_rgb, mat_gloss)
+
+
+def inverse_rendering(freq=71, hidden_seed=5):
+    """The inverse-rendering configuration: Cornell box + light + a displaced
+    geodesic sphere of frequency 71 (20*71^2 = 100,820 triangles; 100,832
+    with the box and light), all diffuse.  Returns (scene data with the
+    hidden per-triangle albedo U[0.2,0.9] (seed 5) and emission 15, the
+    initial parameters: albedo 0.5 and emission 10)."""
+    target = cornell_displaced(geodesic_freq=freq, albedo_seed=hidden_seed, phong_frac=0.0, emission=15.0,
+                               rgb_albedo=True)
+    init_rgb = target.mat_rgb.copy()
+    init_rgb[0] = 10.0
+    init_rgb[4:] = 0.5
+    return target, init_rgb
 
 
 def soup(n=1 << 22, seed=3, lmin=0.002, lmax=0.01):

@@ -51,6 +51,9 @@ def oracle():
         "orc_own_pow": (None, [P, P, LL, P]),
         "orc_bsdf_sample": (None, [I, F, I, P, LL, P]),
         "orc_light_tables": (None, [P, I, P, LL, P, P, P]),
+        "orc_render_adjoint": (None, [P, P, P, U64, P, P, P, P, I]),
+        "orc_render_loss": (ctypes.c_double, [P, P, P, U64, P, I]),
+        "orc_scene_set_rgb": (None, [P, I, P]),
     })
 
 
@@ -176,6 +179,35 @@ class OracleScene:
         self.L.orc_render(self.h, ptr(ip), ptr(fp), int(seed), int(sample_offset), ptr(film),
                           None if smp is None else ptr(smp), threads)
         return (film, smp) if samples else film
+
+    def _params(self, w, h, spp, max_depth, rr_start, rr_prob, nee, precision, camera):
+        from mcrt_amd import scenes
+        cam = camera or scenes.CAMERA
+        ip = np.array([w, h, spp, max_depth, rr_start, 1 if nee else 0, precision, 0, 0], np.int32)
+        fp = np.array([rr_prob, *cam["origin"], *cam["target"], cam["fov_y"]], np.float32)
+        return ip, fp
+
+    def adjoint(self, dfilm, w, h, spp, max_depth, rr_start, rr_prob, seed, nee=True, precision=1, camera=None,
+                threads=8):
+        """(d_albedo, d_emission) per triangle (ntri, 3) float64 and the film."""
+        ip, fp = self._params(w, h, spp, max_depth, rr_start, rr_prob, nee, precision, camera)
+        n = self.d.ntri
+        ga = np.zeros((n, 3), np.float64)
+        ge = np.zeros((n, 3), np.float64)
+        film = np.zeros((h, w, 3), np.float32)
+        dfilm = np.ascontiguousarray(dfilm, np.float32)
+        self.L.orc_render_adjoint(self.h, ptr(ip), ptr(fp), int(seed), ptr(dfilm), ptr(ga), ptr(ge), ptr(film),
+                                  threads)
+        return ga, ge, film
+
+    def loss(self, weights, w, h, spp, max_depth, rr_start, rr_prob, seed, nee=True, camera=None, threads=8):
+        ip, fp = self._params(w, h, spp, max_depth, rr_start, rr_prob, nee, 1, camera)
+        weights = np.ascontiguousarray(weights, np.float32)
+        return self.L.orc_render_loss(self.h, ptr(ip), ptr(fp), int(seed), ptr(weights), threads)
+
+    def set_rgb(self, mat, rgb):
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        self.L.orc_scene_set_rgb(self.h, int(mat), ptr(rgb))
 
     def trace_fixed(self, o, d, nsamples, max_depth, rr_start=100, rr_prob=1.0, seed=1, nee=True, precision=1,
                     threads=8):
