@@ -120,6 +120,47 @@ def render_leg(scene, data, steps, warm, cpu):
     return out
 
 
+ADJOINT = dict(width=512, height=512, spp=32, max_depth=6, rr_start=3, rr_prob=0.8, seed=7)
+
+
+def adjoint_leg(ctx, steps):
+    """Inverse-rendering configuration (Cornell box + displaced geodesic
+    sphere of 100,820 triangles, initial albedo 0.5 / emission 10, L2 loss
+    to a 256-spp target): time of one forward render and of one adjoint
+    (primal recomputation + path replay with gradient scatter)."""
+    import torch
+    from mcrt_amd import scenes
+    A = ADJOINT
+    tgt, init = scenes.inverse_rendering()
+    s = tgt.to_scene(ctx, width=A["width"], height=A["height"])
+    kw = {k: A[k] for k in ("spp", "max_depth", "rr_start", "rr_prob", "seed")}
+    target = torch.empty((A["height"], A["width"], 3), dtype=torch.float32, device="cuda")
+    s.render(target, spp=256, max_depth=A["max_depth"], rr_start=A["rr_start"], rr_prob=A["rr_prob"], seed=1234)
+    s.set_material_rgb(init.astype(np.float32))
+    film = torch.empty_like(target)
+    ga = torch.empty((tgt.ntri, 3), dtype=torch.float32, device="cuda")
+    ge = torch.empty_like(ga)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(torch.cuda.current_stream())
+        for _ in range(steps):
+            fn()
+        e1.record(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    t_fwd = timed(lambda: s.render(film, **kw))
+    dfilm = (2.0 * (film - target) / film.numel()).contiguous()
+    t_adj = timed(lambda: s.render_adjoint(dfilm, ga, ge, **kw))
+    return {"forward_ms": round(t_fwd, 2), "adjoint_ms": round(t_adj, 2),
+            "adjoint_forward_ratio": round(t_adj / t_fwd, 3),
+            "config": dict(A, scene="cornell_geodesic71_displaced", triangles=tgt.ntri,
+                           loss="L2 to 256-spp target")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +169,7 @@ def main():
     ap.add_argument("--rays", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -227,6 +269,10 @@ def main():
         r = render_leg(scene, data, 1 if args.profile else 2, 1, cpu=not (args.no_cpu or args.profile))
         out["render"] = r
         out["msamples_per_s"] = r["msamples_per_s"]
+    if not args.no_adjoint:
+        a = adjoint_leg(ctx, 1 if args.profile else 3)
+        out["adjoint"] = a
+        out["adjoint_forward_ratio"] = a["adjoint_forward_ratio"]
     print(json.dumps(out), flush=True)
 
 
