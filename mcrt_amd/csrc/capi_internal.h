@@ -99,7 +99,7 @@ struct mcrt_scene
     mcrt_build_stats stats{};
     // device residency
     mcrt::DevScene dev;
-    mcrt::DevBuf d_nodes, d_refs, d_shade, d_verts, d_mats, d_lights, d_spill;
+    mcrt::DevBuf d_nodes, d_refs, d_shade, d_verts, d_mats, d_lights, d_spill, d_hot;
     // staging for host-pointer queries
     std::mutex q_mu;
     mcrt::DevBuf q_in, q_out;

@@ -54,7 +54,11 @@ struct alignas(16) TriShade
     float nb[3];
     int32_t light;  // light index or -1
     float nc[3];
-    float pad;
+    int32_t hot;    // gradient slot accumulated per workgroup in LDS, or -1
 };
+
+// Triangles whose gradients the adjoint accumulates in LDS first (lights,
+// then the largest triangles: the ones most paths touch).
+#define MCRT_HOT_SLOTS 64
 static_assert(sizeof(TriShade) == 48, "TriShade must be 48 bytes");
 }  // namespace mcrt

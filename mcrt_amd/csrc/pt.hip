@@ -79,11 +79,38 @@ __device__ __forceinline__ void store_ray(float4* rays, int64_t i, V3f o, V3f d,
 __device__ __forceinline__ V3f f3(float4 v) { return V3f{v.x, v.y, v.z}; }
 __device__ __forceinline__ float4 f4(V3f v) { return make_float4(v.x, v.y, v.z, 0.0f); }
 
-// Wave-reduced scatter-add of a 3-vector into g[3*key]: lanes holding the
-// same key are summed across the wavefront first (up to 4 distinct keys per
-// call -- the lights and the large walls most lanes share), the rest go
-// straight to the hardware float atomics.  Every lane of the wave must call
-// it (the loops below iterate uniformly per wave).
+// Gradient scatter of the adjoint kernels.  Triangles with a hot slot
+// (every light, then the largest triangles: the few keys most path
+// vertices share) are summed per workgroup in LDS and flushed to global
+// memory once per launch; the rest go to the hardware float atomics after
+// lanes of one wavefront holding the same key are merged (camera vertices
+// of one pixel's samples mostly hit the same triangle).
+constexpr int kHotFloats = 3 * MCRT_HOT_SLOTS;  // per gradient array
+
+__device__ __forceinline__ void hot_clear(float* acc)
+{
+    for (int i = threadIdx.x; i < 2 * kHotFloats; i += blockDim.x) acc[i] = 0.0f;
+    __syncthreads();
+}
+
+// acc[0 .. kHotFloats) albedo, acc[kHotFloats ..) emission.
+__device__ __forceinline__ void hot_flush(const DevScene& D, const float* acc, float* g_alb, float* g_em)
+{
+    __syncthreads();
+    const int n = 3 * D.nhot;
+    for (int i = threadIdx.x; i < 2 * n; i += blockDim.x)
+    {
+        const int which = i >= n, r = i - which * n;
+        const float v = acc[which * kHotFloats + r];
+        if (v != 0.0f) unsafeAtomicAdd((which ? g_em : g_alb) + 3 * (int64_t)D.hot_tri[r / 3] + r % 3, v);
+    }
+}
+
+// Wave-merged scatter-add of a 3-vector into g[3*key]: up to 4 groups of
+// lanes with equal keys are summed across the wavefront; once a group
+// holds a single lane (scattered keys) the remaining lanes add directly.
+// Every lane of the wave must call it (the loops below iterate uniformly
+// per wave).
 __device__ __forceinline__ void scatter3(float* g, int key, V3f v, bool valid)
 {
     uint64_t pending = __ballot(valid);
@@ -93,6 +120,7 @@ __device__ __forceinline__ void scatter3(float* g, int key, V3f v, bool valid)
         const int k = __shfl(key, leader);
         const bool mine = valid && key == k;
         const uint64_t same = __ballot(mine);
+        if (__popcll(same) == 1) break;
         float sx = mine ? v.x : 0.0f, sy = mine ? v.y : 0.0f, sz = mine ? v.z : 0.0f;
         for (int off = 32; off > 0; off >>= 1)
         {
@@ -115,6 +143,18 @@ __device__ __forceinline__ void scatter3(float* g, int key, V3f v, bool valid)
         unsafeAtomicAdd(g + 3 * (int64_t)key + 1, v.y);
         unsafeAtomicAdd(g + 3 * (int64_t)key + 2, v.z);
     }
+}
+
+// One gradient term: LDS for hot triangles, scatter3 for the rest.
+__device__ __forceinline__ void grad_add(float* acc, float* g, int key, int hot, V3f v, bool valid)
+{
+    if (valid && hot >= 0)
+    {
+        __hip_atomic_fetch_add(acc + 3 * hot, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(acc + 3 * hot + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(acc + 3 * hot + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    scatter3(g, key, v, valid && hot < 0);
 }
 
 __global__ void k_set_counts(uint32_t* cnt, uint32_t a, uint32_t b)
@@ -168,13 +208,15 @@ __global__ void k_generate(PTState S, PTArgs a)
 template <bool ADJ>
 __global__ void k_resolve(DevScene D, PTState S, int cur, int64_t P)
 {
+    __shared__ float acc[ADJ ? 2 * kHotFloats : 1];
+    if (ADJ) hot_clear(acc);
     const uint32_t nrec = S.cnt[cur][1] / 2;
     const float4* R = S.rec[cur];
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nrec; base += (int64_t)gridDim.x * blockDim.x)
     {
         const int64_t m = base + threadIdx.x;
         const bool act = m < nrec;
-        int em1 = -1, em2 = -1;
+        int em1 = -1, em2 = -1, hot1 = -1, hot2 = -1;
         V3f g1 = v3f(0.0f, 0.0f, 0.0f), g2 = g1;
         if (act)
         {
@@ -206,6 +248,7 @@ __global__ void k_resolve(DevScene D, PTState S, int cur, int64_t P)
                     {
                         le1 = ld3(mt.rgb);
                         em1 = h1.x;
+                        hot1 = ts.hot;
                     }
                 }
             }
@@ -227,6 +270,7 @@ __global__ void k_resolve(DevScene D, PTState S, int cur, int64_t P)
                         {
                             le2 = ld3(mt.rgb);
                             em2 = h2.x;
+                            hot2 = ts.hot;
                         }
                     }
                 }
@@ -245,23 +289,26 @@ __global__ void k_resolve(DevScene D, PTState S, int cur, int64_t P)
         }
         if (ADJ)
         {
-            scatter3(S.g_em, em1, g1, em1 >= 0);
-            scatter3(S.g_em, em2, g2, em2 >= 0);
+            grad_add(acc + kHotFloats, S.g_em, em1, hot1, g1, em1 >= 0);
+            grad_add(acc + kHotFloats, S.g_em, em2, hot2, g2, em2 >= 0);
         }
     }
+    if (ADJ) hot_flush(D, acc, S.g_alb, S.g_em);
 }
 
 // One vertex of every live path (demo_pt.cpp:523-651).
 template <bool ADJ>
 __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
 {
+    __shared__ float acc[ADJ ? 2 * kHotFloats : 1];
+    if (ADJ) hot_clear(acc);
     const int nxt = cur ^ 1;
     const uint32_t A = S.cnt[cur][0];
     const float4* rays = S.rays[cur];
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < A; base += (int64_t)gridDim.x * blockDim.x)
     {
         const int64_t j = base + threadIdx.x;
-        int alb_key = -1, em_key = -1;
+        int alb_key = -1, em_key = -1, hot = -1;
         V3f g_alb = v3f(0.0f, 0.0f, 0.0f), g_em = g_alb;
         do
         {
@@ -288,6 +335,7 @@ __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
                     if (ADJ)
                     {
                         em_key = h.x;
+                        hot = ts.hot;
                         g_em = f3(S.delta[p]) * T;
                     }
                 }
@@ -299,6 +347,7 @@ __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
                 const V3f rest = f3(S.Ltot[p]) - f3(S.L[p]);
                 const V3f dl = f3(S.delta[p]);
                 alb_key = h.x;
+                hot = ts.hot;
                 g_alb = v3f(col.x > 0.0f ? dl.x * rest.x / col.x : 0.0f,
                             col.y > 0.0f ? dl.y * rest.y / col.y : 0.0f,
                             col.z > 0.0f ? dl.z * rest.z / col.z : 0.0f);
@@ -379,10 +428,11 @@ __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
         } while (false);
         if (ADJ)
         {
-            scatter3(S.g_alb, alb_key, g_alb, alb_key >= 0);
-            scatter3(S.g_em, em_key, g_em, em_key >= 0);
+            grad_add(acc, S.g_alb, alb_key, hot, g_alb, alb_key >= 0);
+            grad_add(acc + kHotFloats, S.g_em, em_key, hot, g_em, em_key >= 0);
         }
     }
+    if (ADJ) hot_flush(D, acc, S.g_alb, S.g_em);
 }
 
 // demo_pt.cpp:653 (NaN -> 0) and :703 (clamp >= 0 per channel).

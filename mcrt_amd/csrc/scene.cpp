@@ -2,6 +2,7 @@
 // tables, camera set-up and the query dispatcher.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <fstream>
@@ -149,7 +150,7 @@ int finalize_scene(mcrt_scene* s, int threads)
         }
         t.mat = s->tri_mat[i];
         t.light = -1;
-        t.pad = 0.0f;
+        t.hot = -1;
         if (s->mats[t.mat].type == MCRT_LGHT)
         {
             t.light = (int32_t)lights.size();
@@ -160,6 +161,31 @@ int finalize_scene(mcrt_scene* s, int threads)
             L.area = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) * 0.5f;
             lights.push_back(L);
         }
+    }
+    // Gradient hot slots: every light, then the largest triangles by area.
+    std::vector<int32_t> hot_tri;
+    {
+        int slot = 0;
+        for (const DevLight& L : lights)
+            if (slot < MCRT_HOT_SLOTS) shade[L.tri].hot = slot++;
+        std::vector<std::pair<float, int32_t>> area;
+        area.reserve(n);
+        for (size_t i = 0; i < n; i++)
+            if (shade[i].light < 0)
+            {
+                const Tri& tr = s->tris[i];
+                F3 cr = cross3(sub(tr.b, tr.a), sub(tr.c, tr.a));
+                area.emplace_back(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2], (int32_t)i);
+            }
+        const size_t k = std::min(area.size(), (size_t)std::max(0, MCRT_HOT_SLOTS - slot));
+        std::partial_sort(area.begin(), area.begin() + k, area.end(),
+                          [](const std::pair<float, int32_t>& a, const std::pair<float, int32_t>& b) {
+                              return a.first > b.first || (a.first == b.first && a.second < b.second);
+                          });
+        for (size_t j = 0; j < k; j++) shade[area[j].second].hot = slot++;
+        hot_tri.assign(slot, -1);
+        for (size_t i = 0; i < n; i++)
+            if (shade[i].hot >= 0) hot_tri[shade[i].hot] = (int32_t)i;
     }
     if (!lights.empty())  // DiscreteSampler::update (sample.cpp:103-118)
     {
@@ -181,6 +207,7 @@ int finalize_scene(mcrt_scene* s, int threads)
     if ((rc = s->d_shade.upload(shade.data(), shade.size() * sizeof(TriShade)))) return rc;
     if ((rc = s->d_verts.upload(verts.data(), verts.size() * sizeof(float)))) return rc;
     if ((rc = s->d_lights.upload(lights.data(), lights.size() * sizeof(DevLight)))) return rc;
+    if ((rc = s->d_hot.upload(hot_tri.data(), hot_tri.size() * sizeof(int32_t)))) return rc;
     if ((rc = upload_materials(s))) return rc;
 
     DevScene& d = s->dev;
@@ -190,6 +217,8 @@ int finalize_scene(mcrt_scene* s, int threads)
     d.verts = s->d_verts.as<float>();
     d.lights = s->d_lights.as<DevLight>();
     d.nlights = (int32_t)lights.size();
+    d.hot_tri = s->d_hot.as<int32_t>();
+    d.nhot = (int32_t)hot_tri.size();
     d.stack_bound = wb.stack_bound;
     d.ntri = (int64_t)n;
     d.lds_depth = 16;
@@ -204,7 +233,7 @@ int finalize_scene(mcrt_scene* s, int threads)
     st.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     st.flatten_seconds = std::chrono::duration<double>(t2 - t1).count();
     st.device_bytes = (int64_t)(s->d_nodes.bytes + s->d_refs.bytes + s->d_shade.bytes + s->d_verts.bytes +
-                                s->d_lights.bytes + s->d_mats.bytes + s->d_spill.bytes);
+                                s->d_lights.bytes + s->d_mats.bytes + s->d_spill.bytes + s->d_hot.bytes);
     s->stats = st;
     return MCRT_OK;
 }
