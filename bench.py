@@ -232,6 +232,7 @@ def main():
     hit_frac = float((hits[:, 0] >= 0).float().mean().item())
 
     achieved = bytes_per_ray * n / (ms_step * 1e-3) / 1e9
+    tune = scene.tuning()
     out = {
         "metric": "Mrays/s closest-hit (incoherent, 1.3M-tri displaced icosphere in Cornell box)",
         "value": round(mrays, 1),
@@ -255,7 +256,8 @@ def main():
                      "traffic_source": os.path.relpath(PROFILE, ROOT),
                      "bytes_per_ray": round(bytes_per_ray, 1),
                      "visits_per_ray": {k: round(v, 3) for k, v in per_ray.items()},
-                     "kernel": "k_trace<16,false,false>"},
+                     "kernel": "k_trace<{},closest> x {} waves/CU".format(tune["trace_lds_depth"],
+                                                                          tune["trace_waves_per_cu"])},
         "cpu_baseline": None,
     }
     if not (args.no_cpu or args.profile):

@@ -172,6 +172,33 @@ MCRT_API int mcrt_render_adjoint(mcrt_scene* scene, const mcrt_render_params* pa
                                  const float* dL_dfilm, float* d_albedo, float* d_emission,
                                  float* film, int flags, void* stream);
 
+/* Launch tuning of a scene (no reference counterpart: the reference's CPU
+ * traversal has no launch parameters).  Results never depend on it.  In
+ * mcrt_scene_set_tuning a zero field keeps the current value; it may be
+ * called before or after finalize, but not while work on the scene is in
+ * flight. */
+typedef struct mcrt_tuning {
+    int32_t trace_waves_per_cu; /* resident traversal wavefronts per CU, 1..64 (24)      */
+    int32_t trace_lds_depth;    /* traversal stack entries per lane kept in LDS: 8, 12,
+                                   16 or 24 (12); deeper entries go to a per-stream
+                                   global spill area                                    */
+    int64_t pt_chunk_paths;     /* paths per wavefront chunk of the renders, 1..2^28
+                                   (2^24); path state is ~0.5 KB per path              */
+} mcrt_tuning;
+MCRT_API int mcrt_scene_set_tuning(mcrt_scene* scene, const mcrt_tuning* tuning);
+MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* scene, mcrt_tuning* tuning);
+
+/* Autotuner (no reference counterpart): times closest-hit over the n rays
+ * for trace_waves_per_cu in {8,12,16,20,24,32} x trace_lds_depth in
+ * {8,12,16,24} and keeps the fastest; with `render` != NULL also times one
+ * render per pt_chunk_paths in 2^20..2^24 and keeps the fastest.  The
+ * chosen tuning is applied to the scene and returned in `best`, the best
+ * closest-hit rate in `best_mrays_per_s` (either may be NULL).  Rays follow
+ * `flags` like the queries; synchronous. */
+MCRT_API int mcrt_autotune(mcrt_scene* scene, const float* rays, size_t n,
+                           const mcrt_render_params* render, int flags, void* stream,
+                           mcrt_tuning* best, double* best_mrays_per_s);
+
 #ifdef __cplusplus
 }
 #endif

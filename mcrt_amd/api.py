@@ -47,6 +47,14 @@ class RenderParams(ctypes.Structure):
                 ("nee", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("trace_waves_per_cu", ctypes.c_int32), ("trace_lds_depth", ctypes.c_int32),
+                ("pt_chunk_paths", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib_handle = None
 
 
@@ -80,6 +88,10 @@ def lib():
         "mcrt_query_counters": (I, [P, P, Z, I, P, I, P]),
         "mcrt_render": (I, [P, ctypes.POINTER(RenderParams), P, I, P]),
         "mcrt_render_adjoint": (I, [P, ctypes.POINTER(RenderParams), P, P, P, P, I, P]),
+        "mcrt_scene_set_tuning": (I, [P, ctypes.POINTER(Tuning)]),
+        "mcrt_scene_get_tuning": (I, [P, ctypes.POINTER(Tuning)]),
+        "mcrt_autotune": (I, [P, P, Z, ctypes.POINTER(RenderParams), I, P, ctypes.POINTER(Tuning),
+                              ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -280,6 +292,29 @@ class Scene:
         _check(lib().mcrt_query_counters(self.h, rp, n, int(any_hit), c.ctypes.data, DEVICE_PTRS if rdev else 0,
                                          _stream_of(rdev)))
         return dict(zip(("nodes", "refs", "tris", "rays"), (int(x) for x in c)))
+
+    def set_tuning(self, trace_waves_per_cu=0, trace_lds_depth=0, pt_chunk_paths=0):
+        """Launch tuning (0 keeps a field); results never depend on it."""
+        t = Tuning(trace_waves_per_cu, trace_lds_depth, pt_chunk_paths)
+        _check(lib().mcrt_scene_set_tuning(self.h, ctypes.byref(t)))
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().mcrt_scene_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, rays=None, render=None):
+        """Sweeps the launch tuning on `rays` (closest-hit) and, with
+        render = dict of render parameters, the path-tracer chunk size;
+        applies the fastest.  Returns (tuning dict, best Mrays/s)."""
+        n = 0 if rays is None else rays.shape[0]
+        rp, rdev = _ptr(rays) if n else (None, False)
+        prm = ctypes.byref(self.params(**render)) if render is not None else None
+        t = Tuning()
+        rate = ctypes.c_double()
+        _check(lib().mcrt_autotune(self.h, rp, n, prm, DEVICE_PTRS if rdev else 0, _stream_of(rdev),
+                                   ctypes.byref(t), ctypes.byref(rate)))
+        return t.as_dict(), rate.value
 
     def close(self):
         if self.h:

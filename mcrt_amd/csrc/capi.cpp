@@ -317,13 +317,14 @@ MCRT_API int mcrt_render(mcrt_scene* s, const mcrt_render_params* p, float* film
     int rc = check_render(s, p, "mcrt_render");
     if (rc) return rc;
     HIPCHK(hipSetDevice(s->ctx->device));
-    std::lock_guard<std::mutex> g(s->pt_mu);
     hipStream_t st = (hipStream_t)stream;
-    if (flags & MCRT_DEVICE_PTRS) return render_scene(s, p, film, nullptr, st);
+    StreamRes* r = stream_res(s, st);
+    std::lock_guard<std::mutex> g(r->mu);
+    if (flags & MCRT_DEVICE_PTRS) return render_scene(s, r, p, film, nullptr, st);
     const size_t bytes = (size_t)s->dev_cam.w * s->dev_cam.h * 3 * sizeof(float);
-    if ((rc = s->ptw.film.ensure(bytes))) return rc;
-    if ((rc = render_scene(s, p, s->ptw.film.as<float>(), nullptr, st))) return rc;
-    HIPCHK(hipMemcpyAsync(film, s->ptw.film.p, bytes, hipMemcpyDeviceToHost, st));
+    if ((rc = r->ptw.film.ensure(bytes))) return rc;
+    if ((rc = render_scene(s, r, p, r->ptw.film.as<float>(), nullptr, st))) return rc;
+    HIPCHK(hipMemcpyAsync(film, r->ptw.film.p, bytes, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return MCRT_OK;
 }
@@ -336,21 +337,22 @@ MCRT_API int mcrt_render_adjoint(mcrt_scene* s, const mcrt_render_params* p, con
     int rc = check_render(s, p, "mcrt_render_adjoint");
     if (rc) return rc;
     HIPCHK(hipSetDevice(s->ctx->device));
-    std::lock_guard<std::mutex> g(s->pt_mu);
     hipStream_t st = (hipStream_t)stream;
+    StreamRes* r = stream_res(s, st);
+    std::lock_guard<std::mutex> g(r->mu);
     if (flags & MCRT_DEVICE_PTRS)
     {
         AdjointIO io{dL_dfilm, d_albedo, d_emission};
-        return render_scene(s, p, film, &io, st);
+        return render_scene(s, r, p, film, &io, st);
     }
-    PTWork& w = s->ptw;
+    PTWork& w = r->ptw;
     const size_t fb = (size_t)s->dev_cam.w * s->dev_cam.h * 3 * sizeof(float);
     const size_t gb = (size_t)s->dev.ntri * 3 * sizeof(float);
     if ((rc = w.dfilm.ensure(fb)) || (rc = w.galb.ensure(gb)) || (rc = w.gem.ensure(gb))) return rc;
     if (film && (rc = w.film.ensure(fb))) return rc;
     HIPCHK(hipMemcpyAsync(w.dfilm.p, dL_dfilm, fb, hipMemcpyHostToDevice, st));
     AdjointIO io{w.dfilm.as<float>(), w.galb.as<float>(), w.gem.as<float>()};
-    if ((rc = render_scene(s, p, film ? w.film.as<float>() : nullptr, &io, st))) return rc;
+    if ((rc = render_scene(s, r, p, film ? w.film.as<float>() : nullptr, &io, st))) return rc;
     HIPCHK(hipMemcpyAsync(d_albedo, w.galb.p, gb, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(d_emission, w.gem.p, gb, hipMemcpyDeviceToHost, st));
     if (film) HIPCHK(hipMemcpyAsync(film, w.film.p, fb, hipMemcpyDeviceToHost, st));
@@ -377,10 +379,144 @@ MCRT_API int mcrt_query_counters(mcrt_scene* s, const float* rays, size_t n, int
         r = d_rays.as<float>();
     }
     if ((rc = d_out.ensure(any_hit ? n : n * sizeof(mcrt_hit)))) return rc;
-    HIPCHK(launch_trace(s->dev, r, (int64_t)n, any_hit ? nullptr : d_out.p, any_hit ? d_out.as<uint8_t>() : nullptr,
+    StreamRes* res = stream_res(s, st);
+    std::lock_guard<std::mutex> g(res->mu);
+    DevScene view;
+    if ((rc = launch_view(s, res, view))) return rc;
+    HIPCHK(launch_trace(view, r, (int64_t)n, any_hit ? nullptr : d_out.p, any_hit ? d_out.as<uint8_t>() : nullptr,
                         any_hit ? TRACE_ANY : TRACE_CLOSEST, d_cnt.as<uint64_t>(), st));
     HIPCHK(hipMemcpyAsync(counters, d_cnt.p, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return MCRT_OK;
+}
+
+static int check_tuning(const mcrt_tuning* t)
+{
+    if (t->trace_waves_per_cu < 0 || t->trace_waves_per_cu > 64)
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_waves_per_cu in [1, 64] (0 = keep)");
+    if (t->trace_lds_depth != 0 && !valid_lds_depth(t->trace_lds_depth))
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_lds_depth must be 8, 12, 16 or 24 (0 = keep)");
+    if (t->pt_chunk_paths < 0 || t->pt_chunk_paths > (int64_t(1) << 28))
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: pt_chunk_paths in [1, 2^28] (0 = keep)");
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_scene_set_tuning(mcrt_scene* s, const mcrt_tuning* t)
+{
+    if (!s || !t) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_set_tuning: NULL argument");
+    int rc = check_tuning(t);
+    if (rc) return rc;
+    if (t->trace_waves_per_cu) s->trace_waves_per_cu = t->trace_waves_per_cu;
+    if (t->trace_lds_depth) s->trace_lds_depth = t->trace_lds_depth;
+    if (t->pt_chunk_paths) s->pt_chunk_paths = t->pt_chunk_paths;
+    if (s->finalized) apply_trace_tuning(s);
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* s, mcrt_tuning* t)
+{
+    if (!s || !t) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_get_tuning: NULL argument");
+    t->trace_waves_per_cu = s->trace_waves_per_cu;
+    t->trace_lds_depth = s->trace_lds_depth;
+    t->pt_chunk_paths = s->pt_chunk_paths;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcrt_render_params* render, int flags,
+                           void* stream, mcrt_tuning* best, double* best_mrays_per_s)
+{
+    if (!s || (n && !rays)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_autotune: NULL argument");
+    if (!s->finalized) return fail(MCRT_ERR_STATE, "mcrt_autotune: scene not finalized");
+    if (n == 0 && !render) return fail(MCRT_ERR_INVALID_ARG, "mcrt_autotune: nothing to time (no rays, no render)");
+    int rc;
+    if (render && (rc = check_render(s, render, "mcrt_autotune"))) return rc;
+    HIPCHK(hipSetDevice(s->ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    StreamRes* res = stream_res(s, st);
+    std::lock_guard<std::mutex> g(res->mu);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    struct Ev
+    {
+        hipEvent_t a, b;
+        ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } ev{e0, e1};
+    double best_rate = 0.0;
+    // Traversal: closest-hit over the rays for each (waves per CU, LDS depth).
+    if (n)
+    {
+        DevBuf d_rays, d_hits;
+        const float* r = rays;
+        if (!(flags & MCRT_DEVICE_PTRS))
+        {
+            if ((rc = d_rays.ensure(n * 8 * sizeof(float)))) return rc;
+            HIPCHK(hipMemcpyAsync(d_rays.p, rays, n * 8 * sizeof(float), hipMemcpyHostToDevice, st));
+            r = d_rays.as<float>();
+        }
+        if ((rc = d_hits.ensure(n * sizeof(mcrt_hit)))) return rc;
+        static const int kWaves[] = {8, 12, 16, 20, 24, 32};
+        static const int kDepth[] = {8, 12, 16, 24};
+        int bw = s->trace_waves_per_cu, bd = s->trace_lds_depth;
+        for (int w : kWaves)
+            for (int d : kDepth)
+            {
+                s->trace_waves_per_cu = w;
+                s->trace_lds_depth = d;
+                apply_trace_tuning(s);
+                DevScene view;
+                if ((rc = launch_view(s, res, view))) return rc;
+                HIPCHK(launch_trace(view, r, (int64_t)n, d_hits.p, nullptr, TRACE_CLOSEST, nullptr, st));
+                HIPCHK(hipEventRecord(e0, st));
+                for (int k = 0; k < 3; k++)
+                    HIPCHK(launch_trace(view, r, (int64_t)n, d_hits.p, nullptr, TRACE_CLOSEST, nullptr, st));
+                HIPCHK(hipEventRecord(e1, st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                const double rate = 3.0 * (double)n / (ms * 1e-3) / 1e6;
+                if (rate > best_rate)
+                {
+                    best_rate = rate;
+                    bw = w;
+                    bd = d;
+                }
+            }
+        s->trace_waves_per_cu = bw;
+        s->trace_lds_depth = bd;
+        apply_trace_tuning(s);
+    }
+    // Path tracer: paths per wavefront chunk, one render of the camera each.
+    if (render)
+    {
+        const int64_t total = (int64_t)s->dev_cam.w * s->dev_cam.h * render->spp;
+        DevBuf film;
+        if ((rc = film.ensure((size_t)s->dev_cam.w * s->dev_cam.h * 3 * sizeof(float)))) return rc;
+        int64_t bc = s->pt_chunk_paths;
+        float best_ms = 0.0f;
+        for (int lg = 20; lg <= 24; lg++)
+        {
+            const int64_t c = int64_t(1) << lg;
+            if (lg > 20 && (c >> 1) >= total) break;  // larger chunks change nothing
+            s->pt_chunk_paths = c;
+            if ((rc = render_scene(s, res, render, film.as<float>(), nullptr, st))) return rc;
+            HIPCHK(hipEventRecord(e0, st));
+            if ((rc = render_scene(s, res, render, film.as<float>(), nullptr, st))) return rc;
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (best_ms == 0.0f || ms < best_ms)
+            {
+                best_ms = ms;
+                bc = c;
+            }
+        }
+        s->pt_chunk_paths = bc;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    if (best) mcrt_scene_get_tuning(s, best);
+    if (best_mrays_per_s) *best_mrays_per_s = best_rate;
     return MCRT_OK;
 }
 

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -80,6 +82,19 @@ struct AdjointIO
     float* d_albedo;     // ntri*3
     float* d_emission;   // ntri*3
 };
+
+// Device resources of one stream on one scene.  Work enqueued on distinct
+// streams may overlap on the GPU, so each stream has its own traversal
+// spill stacks and path-tracer workspace; `mu` serialises host threads
+// enqueueing on the same stream (a render's kernels must not interleave
+// with another's on one stream).
+struct StreamRes
+{
+    std::mutex mu;
+    DevBuf spill;
+    PTWork ptw;
+    int64_t pt_capacity = 0;
+};
 }  // namespace mcrt
 
 struct mcrt_scene
@@ -99,15 +114,19 @@ struct mcrt_scene
     mcrt_build_stats stats{};
     // device residency
     mcrt::DevScene dev;
-    mcrt::DevBuf d_nodes, d_refs, d_shade, d_verts, d_mats, d_lights, d_spill, d_hot;
+    mcrt::DevBuf d_nodes, d_refs, d_shade, d_verts, d_mats, d_lights, d_hot;
     // staging for host-pointer queries
     std::mutex q_mu;
     mcrt::DevBuf q_in, q_out;
-    // path tracer
-    std::mutex pt_mu;
-    mcrt::PTWork ptw;
-    int64_t pt_capacity = 0;
-    int64_t pt_chunk_paths = int64_t(1) << 22;
+    // per-stream spill stacks and path-tracer workspaces
+    std::mutex res_mu;
+    std::map<hipStream_t, std::unique_ptr<mcrt::StreamRes>> res;
+    // launch tuning (mcrt_scene_set_tuning / mcrt_autotune)
+    // defaults: the fastest closest-hit setting on MI355X for the 1.3M-triangle
+    // displaced-icosphere scene (24 x 6 KB LDS and 24 x 68 VGPRs fit one CU)
+    int32_t trace_waves_per_cu = 24;
+    int32_t trace_lds_depth = 12;
+    int64_t pt_chunk_paths = int64_t(1) << 24;
 };
 
 namespace mcrt
@@ -117,9 +136,16 @@ int finalize_scene(mcrt_scene* s, int threads);
 int upload_materials(mcrt_scene* s);
 int run_query(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, uint8_t* occ, int mode, int flags,
               hipStream_t stream, uint64_t* counters);
-int pt_workspace(mcrt_scene* s, int64_t P, bool adjoint);
+// The resources of `stream` on `s` (created on first use; live as long as s).
+StreamRes* stream_res(mcrt_scene* s, hipStream_t stream);
+// Sets the traversal tuning of s->dev (LDS depth, spill geometry).
+void apply_trace_tuning(mcrt_scene* s);
+// The scene's device view for a launch with r's spill stacks (caller holds r->mu).
+int launch_view(mcrt_scene* s, StreamRes* r, DevScene& view);
+int pt_workspace(StreamRes* r, int64_t P, bool adjoint);
 // Renders into a device film (W*H*3 floats, may be NULL with adj) on
-// `stream` (asynchronous); with adj also the path-replay gradients.
-int render_scene(mcrt_scene* s, const mcrt_render_params* p, float* film, const AdjointIO* adj,
+// `stream` (asynchronous); with adj also the path-replay gradients.  The
+// caller holds r->mu.
+int render_scene(mcrt_scene* s, StreamRes* r, const mcrt_render_params* p, float* film, const AdjointIO* adj,
                  hipStream_t stream);
 }  // namespace mcrt

@@ -46,9 +46,9 @@ struct DevScene
     int32_t pad0 = 0;
     int64_t ntri = 0;
     uint64_t* spill = nullptr;         // traversal stack overflow storage
-    int64_t spill_lanes = 0;           // lanes the spill buffer covers
+    int64_t spill_lanes = 0;           // lanes the spill buffer covers = resident lanes of a launch
     int32_t spill_depth = 0;           // entries per lane beyond the LDS stack
-    int32_t lds_depth = 16;            // LDS stack entries per lane
+    int32_t lds_depth = 16;            // LDS stack entries per lane (8, 12, 16 or 24)
 };
 
 // Traversal modes.
@@ -65,6 +65,6 @@ enum TraceMode
 hipError_t launch_trace(const DevScene& s, const float* rays, int64_t n, void* hits, uint8_t* occ,
                         int mode, uint64_t* counters, hipStream_t stream);
 
-// Lanes a trace launch keeps resident (the spill buffer must cover them).
-int64_t trace_lanes(int num_cus);
+// LDS stack depths the traversal kernels are instantiated for.
+inline bool valid_lds_depth(int d) { return d == 8 || d == 12 || d == 16 || d == 24; }
 }  // namespace mcrt

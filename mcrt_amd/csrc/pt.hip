@@ -39,7 +39,6 @@ namespace mcrt
 {
 namespace
 {
-constexpr int kLds = 16;
 constexpr int kTPB = 256;
 
 struct PTState
@@ -495,7 +494,7 @@ unsigned grid_for(int64_t n, int num_cus)
 
 // The bounce loop of one chunk (generate .. last resolve).
 template <bool ADJ>
-void bounce_loop(mcrt_scene* s, PTState& S, const PTArgs& a, int max_depth, unsigned g, hipStream_t st)
+void bounce_loop(const DevScene& D, PTState& S, const PTArgs& a, int max_depth, unsigned g, hipStream_t st)
 {
     hipLaunchKernelGGL(k_generate, dim3(g), dim3(kTPB), 0, st, S, a);
     hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[0], (uint32_t)a.P, 0u);
@@ -503,14 +502,13 @@ void bounce_loop(mcrt_scene* s, PTState& S, const PTArgs& a, int max_depth, unsi
     for (int k = 0; k <= max_depth + 1; k++)
     {
         const int nxt = cur ^ 1;
-        hipLaunchKernelGGL((k_trace<kLds, false, false, true>), dim3((unsigned)trace_blocks(s->dev, 3 * a.P)),
-                           dim3(kWave), 0, st, s->dev, S.rays[cur], (int64_t)0, S.cnt[cur], a.P, S.hits,
-                           (uint8_t*)nullptr, (unsigned long long*)nullptr);
-        hipLaunchKernelGGL(k_resolve<ADJ>, dim3(g), dim3(kTPB), 0, st, s->dev, S, cur, a.P);
+        launch_k_trace<false, false, true>(D, trace_blocks(D, 3 * a.P), st, S.rays[cur], (int64_t)0, S.cnt[cur],
+                                           a.P, S.hits, (uint8_t*)nullptr, (unsigned long long*)nullptr);
+        hipLaunchKernelGGL(k_resolve<ADJ>, dim3(g), dim3(kTPB), 0, st, D, S, cur, a.P);
         if (k <= max_depth)
         {
             hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[nxt], 0u, 0u);
-            hipLaunchKernelGGL(k_shade<ADJ>, dim3(g), dim3(kTPB), 0, st, s->dev, S, a, cur, k);
+            hipLaunchKernelGGL(k_shade<ADJ>, dim3(g), dim3(kTPB), 0, st, D, S, a, cur, k);
         }
         cur = nxt;
     }
@@ -518,12 +516,12 @@ void bounce_loop(mcrt_scene* s, PTState& S, const PTArgs& a, int max_depth, unsi
 }  // namespace
 
 // Workspace for up to P paths.
-int pt_workspace(mcrt_scene* s, int64_t P, bool adjoint)
+int pt_workspace(StreamRes* r, int64_t P, bool adjoint)
 {
-    PTWork& w = s->ptw;
+    PTWork& w = r->ptw;
     int rc;
     const size_t n = (size_t)P;
-    if (s->pt_capacity < P)
+    if (r->pt_capacity < P)
     {
         if ((rc = w.T.ensure(n * 16))) return rc;
         if ((rc = w.L.ensure(n * 16))) return rc;
@@ -537,7 +535,7 @@ int pt_workspace(mcrt_scene* s, int64_t P, bool adjoint)
         if ((rc = w.cnt.ensure(4 * sizeof(uint32_t)))) return rc;
         if ((rc = w.hits.ensure(3 * n * 16))) return rc;
         if ((rc = w.samples.ensure(n * 16))) return rc;
-        s->pt_capacity = P;
+        r->pt_capacity = P;
     }
     if (adjoint)
     {
@@ -547,17 +545,20 @@ int pt_workspace(mcrt_scene* s, int64_t P, bool adjoint)
     return MCRT_OK;
 }
 
-int render_scene(mcrt_scene* s, const mcrt_render_params* prm, float* film, const AdjointIO* adj, hipStream_t st)
+int render_scene(mcrt_scene* s, StreamRes* r, const mcrt_render_params* prm, float* film, const AdjointIO* adj,
+                 hipStream_t st)
 {
+    DevScene D;
+    int rc = launch_view(s, r, D);
+    if (rc) return rc;
     const DevCamera& cam = s->dev_cam;
     const int64_t npix = (int64_t)cam.w * cam.h;
     const int spp = prm->spp;
     int64_t pix_per_chunk = s->pt_chunk_paths / spp;
     if (pix_per_chunk < 1) pix_per_chunk = 1;
     if (pix_per_chunk > npix) pix_per_chunk = npix;
-    int rc = pt_workspace(s, pix_per_chunk * spp, adj != nullptr);
-    if (rc) return rc;
-    PTWork& w = s->ptw;
+    if ((rc = pt_workspace(r, pix_per_chunk * spp, adj != nullptr))) return rc;
+    PTWork& w = r->ptw;
     PTState S;
     S.T = w.T.as<float4>();
     S.L = w.L.as<float4>();
@@ -598,11 +599,11 @@ int render_scene(mcrt_scene* s, const mcrt_render_params* prm, float* film, cons
         a.sample_offset = prm->sample_offset;
         a.cam = cam;
         const unsigned g = grid_for(a.P, cus);
-        bounce_loop<false>(s, S, a, prm->max_depth, g, st);
+        bounce_loop<false>(D, S, a, prm->max_depth, g, st);
         if (adj)
         {
             hipLaunchKernelGGL(k_delta, dim3(g), dim3(kTPB), 0, st, S, a, adj->dfilm, scale);
-            bounce_loop<true>(s, S, a, prm->max_depth, g, st);
+            bounce_loop<true>(D, S, a, prm->max_depth, g, st);
         }
         if (film)
         {

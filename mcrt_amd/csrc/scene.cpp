@@ -221,20 +221,42 @@ int finalize_scene(mcrt_scene* s, int threads)
     d.nhot = (int32_t)hot_tri.size();
     d.stack_bound = wb.stack_bound;
     d.ntri = (int64_t)n;
-    d.lds_depth = 16;
-    d.spill_lanes = trace_lanes(s->ctx->num_cus);
-    d.spill_depth = wb.stack_bound > d.lds_depth ? wb.stack_bound - d.lds_depth : 0;
-    if (d.spill_depth > 0)
-    {
-        if ((rc = s->d_spill.ensure((size_t)d.spill_lanes * d.spill_depth * sizeof(uint64_t)))) return rc;
-        d.spill = s->d_spill.as<uint64_t>();
-    }
+    apply_trace_tuning(s);
     auto t2 = clk::now();
     st.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     st.flatten_seconds = std::chrono::duration<double>(t2 - t1).count();
     st.device_bytes = (int64_t)(s->d_nodes.bytes + s->d_refs.bytes + s->d_shade.bytes + s->d_verts.bytes +
-                                s->d_lights.bytes + s->d_mats.bytes + s->d_spill.bytes + s->d_hot.bytes);
+                                s->d_lights.bytes + s->d_mats.bytes + s->d_hot.bytes);
     s->stats = st;
+    return MCRT_OK;
+}
+
+void apply_trace_tuning(mcrt_scene* s)
+{
+    DevScene& d = s->dev;
+    d.lds_depth = s->trace_lds_depth;
+    d.spill_lanes = (int64_t)s->ctx->num_cus * s->trace_waves_per_cu * 64;
+    d.spill_depth = d.stack_bound > d.lds_depth ? d.stack_bound - d.lds_depth : 0;
+    d.spill = nullptr;  // per stream, see launch_view
+}
+
+StreamRes* stream_res(mcrt_scene* s, hipStream_t stream)
+{
+    std::lock_guard<std::mutex> g(s->res_mu);
+    std::unique_ptr<StreamRes>& r = s->res[stream];
+    if (!r) r.reset(new StreamRes());
+    return r.get();
+}
+
+int launch_view(mcrt_scene* s, StreamRes* r, DevScene& view)
+{
+    view = s->dev;
+    if (view.spill_depth > 0)
+    {
+        int rc = r->spill.ensure((size_t)view.spill_lanes * view.spill_depth * sizeof(uint64_t));
+        if (rc) return rc;
+        view.spill = r->spill.as<uint64_t>();
+    }
     return MCRT_OK;
 }
 
@@ -244,19 +266,23 @@ int run_query(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, uint8_
     if (!s->finalized) return fail(MCRT_ERR_STATE, "ray query before mcrt_scene_finalize");
     if (n == 0) return MCRT_OK;
     HIPCHK(hipSetDevice(s->ctx->device));
+    StreamRes* r = stream_res(s, stream);
+    std::lock_guard<std::mutex> gr(r->mu);
+    DevScene view;
+    int rc = launch_view(s, r, view);
+    if (rc) return rc;
     if (flags & MCRT_DEVICE_PTRS)
     {
-        HIPCHK(launch_trace(s->dev, rays, (int64_t)n, hits, occ, mode, counters, stream));
+        HIPCHK(launch_trace(view, rays, (int64_t)n, hits, occ, mode, counters, stream));
         return MCRT_OK;
     }
     std::lock_guard<std::mutex> g(s->q_mu);
     const size_t in_bytes = n * 8 * sizeof(float);
     const size_t out_bytes = mode == TRACE_ANY ? n : n * sizeof(mcrt_hit);
-    int rc;
     if ((rc = s->q_in.ensure(in_bytes))) return rc;
     if ((rc = s->q_out.ensure(out_bytes))) return rc;
     HIPCHK(hipMemcpyAsync(s->q_in.p, rays, in_bytes, hipMemcpyHostToDevice, stream));
-    HIPCHK(launch_trace(s->dev, s->q_in.as<float>(), (int64_t)n, mode == TRACE_ANY ? nullptr : s->q_out.p,
+    HIPCHK(launch_trace(view, s->q_in.as<float>(), (int64_t)n, mode == TRACE_ANY ? nullptr : s->q_out.p,
                         mode == TRACE_ANY ? s->q_out.as<uint8_t>() : nullptr, mode, counters, stream));
     HIPCHK(hipMemcpyAsync(mode == TRACE_ANY ? (void*)occ : (void*)hits, s->q_out.p, out_bytes,
                           hipMemcpyDeviceToHost, stream));

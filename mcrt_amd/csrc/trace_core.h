@@ -119,7 +119,7 @@ struct TraceStack
     float* lo;
     int stride, lane, depth;
     uint64_t* spill;  // global, [spill_lane * spill_depth + (k - depth)]
-    int64_t spill_lane;
+    uint32_t spill_lane;
     int spill_depth;
 
     MCRT_HD void push(int sp, uint32_t n, float l)
@@ -131,7 +131,7 @@ struct TraceStack
         }
         else
         {
-            spill[spill_lane * spill_depth + (sp - depth)] =
+            spill[(size_t)spill_lane * spill_depth + (sp - depth)] =
                 ((uint64_t)bits_of(l) << 32) | (uint64_t)n;
         }
     }
@@ -144,7 +144,7 @@ struct TraceStack
         }
         else
         {
-            uint64_t x = spill[spill_lane * spill_depth + (sp - depth)];
+            uint64_t x = spill[(size_t)spill_lane * spill_depth + (sp - depth)];
             n = (uint32_t)x;
             l = float_of((uint32_t)(x >> 32));
         }
@@ -156,28 +156,37 @@ struct TraceCount
     uint32_t nodes = 0, refs = 0, tris = 0;
 };
 
+// The traversal of one ray as a sequence of steps (one wide node or one
+// leaf per step), so a kernel can hand a lane its next ray as soon as the
+// current one ends instead of waiting for the wavefront's slowest lane.
 template <bool ANY, bool COUNT>
-MCRT_HD int trace(const DevScene& S, TraceStack& st, const RayPrep& r, float tmax,
-                                     float& t_out, float& u_out, float& v_out, TraceCount& cnt)
+struct Tracer
 {
-    int prim = -1;
-    float R1 = tmax, bu = 0.0f, bv = 0.0f;
-    uint32_t cur = 0;
-    int sp = 0;
-    if (S.nodes == nullptr)
+    RayPrep r;
+    float R1, bu, bv;
+    int prim;
+    uint32_t cur;
+    int sp;
+
+    MCRT_HD void start(const RayPrep& rp, float tmax)
     {
-        t_out = __builtin_inff();
-        u_out = v_out = 0.0f;
-        return -1;
+        r = rp;
+        R1 = tmax;
+        bu = bv = 0.0f;
+        prim = -1;
+        cur = 0;
+        sp = 0;
     }
-    while (true)
+
+    // Visits `cur`; returns true once the traversal has ended.
+    MCRT_HD bool step(const DevScene& S, TraceStack& st, TraceCount& cnt)
     {
         if (!(cur & MCRT_CHILD_LEAF))
         {
             const float4* p = reinterpret_cast<const float4*>(S.nodes + cur);
             const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
             const uint4 ch = reinterpret_cast<const uint4*>(p)[6];
-            const uint4 meta = reinterpret_cast<const uint4*>(p)[7];
+            const uint2 meta = reinterpret_cast<const uint2*>(p)[14];  // order_lo, order_hi
             if (COUNT) cnt.nodes++;
             float l0, l1, l2, l3;
             bool p0 = ch.x != MCRT_CHILD_EMPTY && slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, R1, l0);
@@ -202,7 +211,7 @@ MCRT_HD int trace(const DevScene& S, TraceStack& st, const RayPrep& r, float tma
             if (next != MCRT_CHILD_EMPTY)
             {
                 cur = next;
-                continue;
+                return false;
             }
         }
         else
@@ -225,17 +234,10 @@ MCRT_HD int trace(const DevScene& S, TraceStack& st, const RayPrep& r, float tma
                     bu = u;
                     bv = v;
                     prim = (int)bits_of(d.w);
-                    if (ANY)
-                    {
-                        t_out = t;
-                        u_out = u;
-                        v_out = v;
-                        return prim;
-                    }
+                    if (ANY) return true;
                 }
             }
         }
-        bool found = false;
         while (sp > 0)
         {
             --sp;
@@ -245,23 +247,38 @@ MCRT_HD int trace(const DevScene& S, TraceStack& st, const RayPrep& r, float tma
             if (!(lo > R1))
             {
                 cur = n;
-                found = true;
-                break;
+                return false;
             }
         }
-        if (!found) break;
+        return true;
     }
-    if (prim < 0)
+
+    // Primitive id (-1 on a miss) and t, u, v (inf, 0, 0 on a miss).
+    MCRT_HD int result(float& t_out, float& u_out, float& v_out) const
     {
-        t_out = __builtin_inff();
-        u_out = v_out = 0.0f;
+        if (prim < 0)
+        {
+            t_out = __builtin_inff();
+            u_out = v_out = 0.0f;
+        }
+        else
+        {
+            t_out = R1;
+            u_out = bu;
+            v_out = bv;
+        }
+        return prim;
     }
-    else
-    {
-        t_out = R1;
-        u_out = bu;
-        v_out = bv;
-    }
-    return prim;
+};
+
+template <bool ANY, bool COUNT>
+MCRT_HD int trace(const DevScene& S, TraceStack& st, const RayPrep& r, float tmax,
+                  float& t_out, float& u_out, float& v_out, TraceCount& cnt)
+{
+    Tracer<ANY, COUNT> tr;
+    tr.start(r, tmax);
+    if (S.nodes != nullptr)
+        while (!tr.step(S, st, cnt)) {}
+    return tr.result(t_out, u_out, v_out);
 }
 }  // namespace mcrt

@@ -1,8 +1,8 @@
 // The traversal kernel template, shared by the query entry points
 // (traverse.hip) and the path tracer's extend step (pt.hip).
 //
-// One lane per ray, grid-strided over the batch, one wavefront per
-// workgroup (its LDS stack slab is private to the wave).  Two ray sources:
+// One lane per ray at a time, grid-strided over the batch, one wavefront
+// per workgroup (its LDS stack slab is private to the wave).  Two ray sources:
 //   * SEG = false: rays[0, n) with n known to the host;
 //   * SEG = true : two segments whose sizes live in device memory (the
 //     path tracer's compacted continuation rays [0, cnt[0]) and its
@@ -30,9 +30,11 @@ __global__ __launch_bounds__(kWave) void k_trace(DevScene S, const float4* __res
     st.lane = threadIdx.x;
     st.depth = D;
     st.spill = S.spill;
-    st.spill_lane = (int64_t)blockIdx.x * kWave + threadIdx.x;
+    st.spill_lane = blockIdx.x * kWave + threadIdx.x;
     st.spill_depth = S.spill_depth;
-    int64_t n0 = n, total = n;
+    // 32-bit ray indices: a launch covers fewer than 2^31 rays (launch_trace
+    // splits larger batches; path-tracer chunks are at most 3 * 2^28 rays).
+    uint32_t n0 = (uint32_t)n, total = (uint32_t)n;
     if (SEG)
     {
         n0 = seg_cnt[0];
@@ -40,18 +42,38 @@ __global__ __launch_bounds__(kWave) void k_trace(DevScene S, const float4* __res
     }
     TraceCount cnt;
     uint32_t nrays = 0;
-    for (int64_t k = (int64_t)blockIdx.x * kWave + threadIdx.x; k < total; k += (int64_t)gridDim.x * kWave)
+    const uint32_t stride = gridDim.x * kWave;
+    uint32_t k = blockIdx.x * kWave + threadIdx.x;
+    // Lane-persistent loop: each lane walks its rays k, k + stride, ... and
+    // starts the next one in the same iteration in which the previous ends,
+    // so divergent ray lengths do not idle the wavefront.
+    Tracer<ANY, COUNT> tr;
+    uint32_t i = 0;
+    bool live = k < total;
+    if (live)
     {
-        const int64_t i = (SEG && k >= n0) ? seg1 + (k - n0) : k;
-        const float4 a = rays[2 * i], b = rays[2 * i + 1];
-        RayPrep r = prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w);
+        i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : k;
+        const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+        tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w);
+    }
+    while (live)
+    {
+        if (S.nodes != nullptr && !tr.step(S, st, cnt)) continue;
         float t, u, v;
-        int prim = trace<ANY, COUNT>(S, st, r, b.w, t, u, v, cnt);
+        const int prim = tr.result(t, u, v);
         if (ANY)
             occ[i] = prim >= 0 ? 1 : 0;
         else
             hits[i] = make_int4(prim, (int)bits_of(t), (int)bits_of(u), (int)bits_of(v));
         nrays++;
+        k += stride;
+        live = k < total;
+        if (live)
+        {
+            i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : k;
+            const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+            tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w);
+        }
     }
     if (COUNT)
     {
@@ -71,5 +93,32 @@ inline int64_t trace_blocks(const DevScene& s, int64_t n)
     if (blocks > max_blocks) blocks = max_blocks;
     if (blocks < 1) blocks = 1;
     return blocks;
+}
+
+// Launches k_trace with the scene's LDS stack depth on `blocks` workgroups.
+template <bool ANY, bool COUNT, bool SEG>
+inline void launch_k_trace(const DevScene& S, int64_t blocks, hipStream_t st, const float4* rays, int64_t n,
+                           const uint32_t* seg_cnt, int64_t seg1, int4* hits, uint8_t* occ,
+                           unsigned long long* counters)
+{
+    const dim3 g((unsigned)blocks), b(kWave);
+    switch (S.lds_depth)
+    {
+    case 8:
+        hipLaunchKernelGGL((k_trace<8, ANY, COUNT, SEG>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ, counters);
+        break;
+    case 12:
+        hipLaunchKernelGGL((k_trace<12, ANY, COUNT, SEG>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+                           counters);
+        break;
+    case 24:
+        hipLaunchKernelGGL((k_trace<24, ANY, COUNT, SEG>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+                           counters);
+        break;
+    default:
+        hipLaunchKernelGGL((k_trace<16, ANY, COUNT, SEG>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+                           counters);
+        break;
+    }
 }
 }  // namespace mcrt

@@ -5,31 +5,34 @@ namespace mcrt
 {
 namespace
 {
-constexpr int kLdsDepth = 16;
-constexpr int kBlocksPerCU = 16;
-
 template <bool ANY, bool COUNT>
 hipError_t launch(const DevScene& s, const float* rays, int64_t n, void* hits, uint8_t* occ, uint64_t* counters,
                   hipStream_t stream)
 {
-    hipLaunchKernelGGL((k_trace<kLdsDepth, ANY, COUNT, false>), dim3((unsigned)trace_blocks(s, n)), dim3(kWave), 0,
-                       stream, s, reinterpret_cast<const float4*>(rays), n, nullptr, (int64_t)0,
-                       reinterpret_cast<int4*>(hits), occ, reinterpret_cast<unsigned long long*>(counters));
+    launch_k_trace<ANY, COUNT, false>(s, trace_blocks(s, n), stream, reinterpret_cast<const float4*>(rays), n,
+                                      nullptr, (int64_t)0, reinterpret_cast<int4*>(hits), occ,
+                                      reinterpret_cast<unsigned long long*>(counters));
     return hipGetLastError();
 }
 }  // namespace
 
-int64_t trace_lanes(int num_cus) { return (int64_t)num_cus * kBlocksPerCU * kWave; }
-
 hipError_t launch_trace(const DevScene& s, const float* rays, int64_t n, void* hits, uint8_t* occ, int mode,
                         uint64_t* counters, hipStream_t stream)
 {
-    if (n <= 0) return hipSuccess;
     const bool any = mode == TRACE_ANY;
-    if (counters)
-        return any ? launch<true, true>(s, rays, n, hits, occ, counters, stream)
-                   : launch<false, true>(s, rays, n, hits, occ, counters, stream);
-    return any ? launch<true, false>(s, rays, n, hits, occ, counters, stream)
-               : launch<false, false>(s, rays, n, hits, occ, counters, stream);
+    const int64_t kMaxLaunch = int64_t(1) << 30;  // the kernel indexes rays with 32 bits
+    for (int64_t off = 0; off < n; off += kMaxLaunch)
+    {
+        const int64_t m = n - off < kMaxLaunch ? n - off : kMaxLaunch;
+        const float* r = rays + 8 * off;
+        void* h = any ? nullptr : static_cast<void*>(static_cast<int4*>(hits) + off);
+        uint8_t* o = any ? occ + off : nullptr;
+        hipError_t e = counters ? (any ? launch<true, true>(s, r, m, h, o, counters, stream)
+                                       : launch<false, true>(s, r, m, h, o, counters, stream))
+                                : (any ? launch<true, false>(s, r, m, h, o, counters, stream)
+                                       : launch<false, false>(s, r, m, h, o, counters, stream));
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 }  // namespace mcrt
