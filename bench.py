@@ -161,6 +161,46 @@ def adjoint_leg(ctx, steps):
                            loss="L2 to 256-spp target")}
 
 
+def c3_leg(ctx, steps):
+    """Ray-query microbenchmark: 4,194,304-triangle soup, 16,777,216
+    incoherent and 16,777,216 coherent rays, closest hit and any hit with
+    tfar = 0.5 (Mrays/s each, one launch per step)."""
+    import torch
+    import mcrt_amd
+    from mcrt_amd import scenes
+    tris = scenes.c3_soup()
+    s = mcrt_amd.Scene(ctx)
+    s.add(tris, s.add_material(mcrt_amd.DIFF, (0.5, 0.5, 0.5)))
+    st = s.finalize()
+    log(f"C3 soup SBVH: {st['build_seconds']:.2f} s build, {st['refs']} refs")
+    out = {"config": {"triangles": int(tris.shape[0]), "rays": scenes.C3_RAYS, "refs": st["refs"],
+                      "wide_nodes": st["wide_nodes"], "sbvh_build_s": round(st["build_seconds"], 2),
+                      "any_hit_tfar": scenes.C3_ANY_TFAR}}
+    n = scenes.C3_RAYS
+    hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+    occ = torch.empty(n, dtype=torch.uint8, device="cuda")
+    for kind in ("incoherent", "coherent"):
+        for any_hit in (False, True):
+            rays = torch.from_numpy(scenes.c3_rays(kind, any_hit=any_hit)).cuda()
+            fn = (lambda: s.occluded(rays, occ)) if any_hit else (lambda: s.intersect(rays, hits))
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(torch.cuda.current_stream())
+            for _ in range(steps):
+                fn()
+            e1.record(torch.cuda.current_stream())
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            key = f"{kind}_{'any_hit' if any_hit else 'closest'}"
+            frac = float(occ.float().mean()) if any_hit else float((hits[:, 0] >= 0).float().mean())
+            out[key] = {"mrays_per_s": round(n / ms / 1e3, 1), "ms": round(ms, 3),
+                        ("occluded_fraction" if any_hit else "hit_fraction"): round(frac, 4)}
+            del rays
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +210,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 4M-triangle soup ray-query leg")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -271,6 +312,8 @@ def main():
         r = render_leg(scene, data, 1 if args.profile else 2, 1, cpu=not (args.no_cpu or args.profile))
         out["render"] = r
         out["msamples_per_s"] = r["msamples_per_s"]
+    if not (args.no_c3 or args.profile):
+        out["c3_soup"] = c3_leg(ctx, 5)
     if not args.no_adjoint:
         a = adjoint_leg(ctx, 1 if args.profile else 3)
         out["adjoint"] = a

@@ -318,6 +318,32 @@ def pinhole_rays(width, height, origin=(0.5, 0.5, -1.5), target=(0.5, 0.5, 0.5),
     return r
 
 
+C3_TRIS = 1 << 22           # "4M" triangles, 2^22 so that 16M rays = 4096^2
+C3_RAYS = 1 << 24
+C3_ANY_TFAR = 0.5           # occluded(ray, tfar): tmax = tfar - 1e-4 (scenebvh.cpp:384)
+
+
+def c3_soup():
+    """Ray-query microbenchmark scene: 4,194,304 soup triangles (seed 3)."""
+    return soup(C3_TRIS, seed=3)
+
+
+def c3_rays(kind, any_hit=False):
+    """16,777,216 rays of the soup microbenchmark: 'incoherent' (origins
+    U[0,1]^3, directions uniform on the sphere, seed 3) or 'coherent'
+    (4096^2 pinhole grid from (0.5, 0.5, -1.5) towards the cube centre,
+    fov 45, pixel centres).  any_hit: tmax = float(0.5) - float(1e-4)."""
+    if kind == "incoherent":
+        r = incoherent_rays(C3_RAYS, lo=np.zeros(3), hi=np.ones(3), seed=3)
+    elif kind == "coherent":
+        r = pinhole_rays(4096, 4096)
+    else:
+        raise ValueError(kind)
+    if any_hit:
+        r[:, 7] = np.float32(C3_ANY_TFAR) - np.float32(1e-4)
+    return r
+
+
 def write_obj(path, tris):
     """Writes triangles as an OBJ of `v` and `f a b c` lines (no vertex sharing)."""
     t = np.asarray(tris, np.float32).reshape(-1, 3)
