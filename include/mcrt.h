@@ -172,6 +172,18 @@ MCRT_API int mcrt_render_adjoint(mcrt_scene* scene, const mcrt_render_params* pa
                                  const float* dL_dfilm, float* d_albedo, float* d_emission,
                                  float* film, int flags, void* stream);
 
+/* Film output (host memory, W*H*3 floats, row y = 0 at the image bottom);
+ * byte-identical to the reference's FrameBuffer writers on finite,
+ * non-negative films (NaN or negative channels are written as 0, where the
+ * reference's conversion is undefined).  No device or context is needed.
+ *   mcrt_film_write_hdr     FrameBuffer::dumpHDR       src/core/film.cpp:65-108
+ *   mcrt_film_write_ppm     FrameBuffer::dumpPPM       src/core/film.cpp:9-33
+ *   mcrt_film_tonemap_gamma FrameBuffer::tonemapGamma  src/core/film.cpp:195-202
+ *                           (in place, n floats)                               */
+MCRT_API int mcrt_film_write_hdr(const char* path, const float* film, int32_t width, int32_t height);
+MCRT_API int mcrt_film_write_ppm(const char* path, const float* film, int32_t width, int32_t height);
+MCRT_API int mcrt_film_tonemap_gamma(float* film, size_t n, float gamma);
+
 /* Launch tuning of a scene (no reference counterpart: the reference's CPU
  * traversal has no launch parameters).  Results never depend on it.  In
  * mcrt_scene_set_tuning a zero field keeps the current value; it may be

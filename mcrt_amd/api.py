@@ -88,6 +88,9 @@ def lib():
         "mcrt_query_counters": (I, [P, P, Z, I, P, I, P]),
         "mcrt_render": (I, [P, ctypes.POINTER(RenderParams), P, I, P]),
         "mcrt_render_adjoint": (I, [P, ctypes.POINTER(RenderParams), P, P, P, P, I, P]),
+        "mcrt_film_write_hdr": (I, [ctypes.c_char_p, P, ctypes.c_int32, ctypes.c_int32]),
+        "mcrt_film_write_ppm": (I, [ctypes.c_char_p, P, ctypes.c_int32, ctypes.c_int32]),
+        "mcrt_film_tonemap_gamma": (I, [P, Z, ctypes.c_float]),
         "mcrt_scene_set_tuning": (I, [P, ctypes.POINTER(Tuning)]),
         "mcrt_scene_get_tuning": (I, [P, ctypes.POINTER(Tuning)]),
         "mcrt_autotune": (I, [P, P, Z, ctypes.POINTER(RenderParams), I, P, ctypes.POINTER(Tuning),
@@ -326,6 +329,33 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+def _host_film(film):
+    f = np.ascontiguousarray(film, np.float32)
+    if f.ndim != 3 or f.shape[2] != 3:
+        raise ValueError("film must be (H, W, 3)")
+    return f
+
+
+def write_hdr(path, film):
+    """Radiance .hdr of an (H, W, 3) film, row 0 = image bottom
+    (FrameBuffer::dumpHDR layout)."""
+    f = _host_film(film)
+    _check(lib().mcrt_film_write_hdr(os.fsencode(path), f.ctypes.data, f.shape[1], f.shape[0]))
+
+
+def write_ppm(path, film):
+    """Binary PPM of an (H, W, 3) film in [0, 1] (FrameBuffer::dumpPPM)."""
+    f = _host_film(film)
+    _check(lib().mcrt_film_write_ppm(os.fsencode(path), f.ctypes.data, f.shape[1], f.shape[0]))
+
+
+def tonemap_gamma(film, gamma=2.2):
+    """FrameBuffer::tonemapGamma on a copy of the film."""
+    f = _host_film(film).copy()
+    _check(lib().mcrt_film_tonemap_gamma(f.ctypes.data, f.size, gamma))
+    return f
 
 
 def split_hits(hits):

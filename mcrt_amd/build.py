@@ -19,8 +19,10 @@ LIB = os.path.join(HERE, "libmcrt_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("MCRT_ARCH", "gfx950")
 
-HOST_SRCS = ["capi.cpp", "scene.cpp", "sbvh.cpp", "wide.cpp"]
+HOST_SRCS = ["capi.cpp", "scene.cpp", "sbvh.cpp", "wide.cpp", "film.cpp"]
 HIP_SRCS = ["traverse.hip", "pt.hip"]
+DEMOS = ["demo_pt"]                 # C++ programs on the facade include/mcrt.hpp
+BIN = os.path.join(HERE, "bin")
 
 INC = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-I" + os.path.join(ROCM, "include")]
 HOST_FLAGS = ["-std=c++17", "-O3", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fvisibility=hidden",
@@ -62,6 +64,11 @@ def build(verbose=False, jobs=8):
     _run(["g++", "-shared", "-fopenmp", "-o", tmp] + objs + ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
                                                              "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
+    os.makedirs(BIN, exist_ok=True)
+    for d in DEMOS:
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+              os.path.join(HERE, "demo", d + ".cpp"), "-o", os.path.join(BIN, d), "-L" + HERE, "-lmcrt_hip",
+              "-Wl,-rpath,$ORIGIN/.."])
     return LIB
 
 

@@ -30,7 +30,10 @@ _WALLS = {
 _LIGHT_Y = np.float32(548.79999) - np.float32(0.5)
 _LIGHT = [(343, _LIGHT_Y, 227), (343, _LIGHT_Y, 332), (213, _LIGHT_Y, 332), (213, _LIGHT_Y, 227)]
 
-CAMERA = dict(origin=(27.8, 27.3, -80.0), target=(27.8, 27.3, -79.9), fov_y=39.3077)
+# Camera of the reference demo: vec3(278, 273, -800) * 0.1f looking at
+# vec3(278, 273, -799) * 0.1f (demo_pt.cpp:88-92), as the float products.
+CAMERA = dict(origin=(27.80000114440918, 27.30000114440918, -80.0),
+              target=(27.80000114440918, 27.30000114440918, -79.9000015258789), fov_y=39.3077)
 BOX_CENTER = np.array([27.8, 27.439999, 27.96], np.float64)
 SPHERE_RADIUS = 12.0
 

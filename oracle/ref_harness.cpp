@@ -20,6 +20,7 @@
 #include "camera.h"
 #include "constants.h"
 #include "debug_mesh.h"
+#include "film.h"
 #include "material.h"
 #include "mesh.h"
 #include "ray.h"
@@ -306,4 +307,43 @@ REF_API void ref_discrete(const float* data, int n, const float* r, long long nr
     DiscreteSampler s(d);
     for (int i = 0; i < n; i++) pdf[i] = s.pdf(i);
     for (long long k = 0; k < nr; k++) idx[k] = s.sample(r[k]);
+}
+
+// FrameBuffer holding a W x H x 3 film (row y = 0 at the bottom), filled
+// through accumulatePixel.
+static void fill_film(FrameBuffer& fb, const float* film, int w, int h)
+{
+    fb.resize(w, h);
+    for (int j = 0; j < h; j++)
+        for (int i = 0; i < w; i++)
+        {
+            const float* p = film + 3 * ((size_t)j * w + i);
+            fb.accumulatePixel(i, j, vec3(p[0], p[1], p[2]));
+        }
+}
+
+// FrameBuffer::dumpHDR / dumpPPM of `film` to `path`.
+REF_API void ref_film_dump(const char* path, const float* film, int w, int h, int ppm)
+{
+    FrameBuffer fb;
+    fill_film(fb, film, w, h);
+    if (ppm)
+        fb.dumpPPM(path);
+    else
+        fb.dumpHDR(path);
+}
+
+// FrameBuffer::tonemapGamma in place on `film`.
+REF_API void ref_film_tonemap_gamma(float* film, int w, int h, float gamma)
+{
+    FrameBuffer fb;
+    fill_film(fb, film, w, h);
+    fb.tonemapGamma(gamma);
+    for (int j = 0; j < h; j++)
+        for (int i = 0; i < w; i++)
+        {
+            vec3 c = fb.pixel(i, j);
+            float* p = film + 3 * ((size_t)j * w + i);
+            p[0] = c.x; p[1] = c.y; p[2] = c.z;
+        }
 }

@@ -72,6 +72,8 @@ def reference():
         "ref_camera_rays": (None, [P, P, I, I, F, P, LL, P]),
         "ref_bsdf": (None, [I, P, P, LL, P]),
         "ref_discrete": (None, [P, I, P, LL, P, P]),
+        "ref_film_dump": (None, [ctypes.c_char_p, P, I, I, I]),
+        "ref_film_tonemap_gamma": (None, [P, I, I, ctypes.c_float]),
     })
 
 

@@ -118,16 +118,7 @@ def main():
                         input_sha=sha(box, rays, tris, slab, axis))
     print("primitives", ov.mean(), hit.mean())
 
-    # Camera::makeRay at pixel centres and corners of a 64x48 film
-    from mcrt_amd import scenes
-    cam = scenes.CAMERA
-    o = np.array(cam["origin"], np.float32)
-    t = np.array(cam["target"], np.float32)
-    xs, ys = np.meshgrid(np.arange(0, 65, 4, dtype=np.float32), np.arange(0, 49, 4, dtype=np.float32))
-    pxy = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.float32)
-    cr = np.zeros((pxy.shape[0], 6), np.float32)
-    R.ref_camera_rays(ptr(o), ptr(t), 64, 48, cam["fov_y"], ptr(pxy), pxy.shape[0], ptr(cr))
-    np.savez_compressed(os.path.join(HERE, "camera.npz"), pxy=pxy, rays=cr)
+    camera_fixture(R)
 
     # BSDF sample / pdf / evaluate (material.cpp:102-127)
     rng = np.random.default_rng(11)
@@ -165,5 +156,57 @@ def main():
     print("obj faces", nf)
 
 
+def camera_fixture(R):
+    """Camera::makeRay at pixel centres and corners of a 64x48 film."""
+    from mcrt_amd import scenes
+    cam = scenes.CAMERA
+    o = np.array(cam["origin"], np.float32)
+    t = np.array(cam["target"], np.float32)
+    xs, ys = np.meshgrid(np.arange(0, 65, 4, dtype=np.float32), np.arange(0, 49, 4, dtype=np.float32))
+    pxy = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.float32)
+    cr = np.zeros((pxy.shape[0], 6), np.float32)
+    R.ref_camera_rays(ptr(o), ptr(t), 64, 48, cam["fov_y"], ptr(pxy), pxy.shape[0], ptr(cr))
+    np.savez_compressed(os.path.join(HERE, "camera.npz"), pxy=pxy, rays=cr)
+
+
+def film_inputs(seed=13, w=131, h=7):
+    """(H, W, 3) film spanning the RGBE/PPM edge cases: zeros, values below
+    the 1e-32 cut, denormal-free tiny values, [0, 1], above 1, powers of
+    two; W = 131 makes every RGBE plane split into a 127-byte and a 4-byte
+    literal run."""
+    rng = np.random.default_rng(seed)
+    f = rng.uniform(0.0, 1.0, (h, w, 3)).astype(np.float32)
+    f[0] *= np.float32(8.0)
+    f[1] = rng.uniform(0.0, 1e6, (w, 3)).astype(np.float32)
+    f[2, : w // 2] = 0.0
+    f[2, w // 2:] = np.float32(1e-33)
+    f[3] = (2.0 ** rng.integers(-20, 20, (w, 3))).astype(np.float32)
+    f[4] = rng.uniform(0.0, 1e-30, (w, 3)).astype(np.float32)
+    f[5, :, 1:] = 0.0
+    return f
+
+
+def film_fixture(R):
+    import tempfile
+    f = film_inputs()
+    h, w, _ = f.shape
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for name, ppm in (("hdr", 0), ("ppm", 1)):
+            path = os.path.join(d, "f." + name)
+            R.ref_film_dump(path.encode(), ptr(f), w, h, ppm)
+            out[name] = np.frombuffer(open(path, "rb").read(), np.uint8)
+    g = f.copy()
+    R.ref_film_tonemap_gamma(ptr(g), w, h, 2.2)
+    np.savez_compressed(os.path.join(HERE, "film.npz"), film=f, hdr=out["hdr"], ppm=out["ppm"], gamma22=g)
+    print("film", {k: v.size for k, v in out.items()})
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["film"]:
+        film_fixture(_libs.reference())
+    elif sys.argv[1:] == ["camera"]:
+        camera_fixture(_libs.reference())
+    else:
+        main()
+        film_fixture(_libs.reference())
