@@ -50,11 +50,15 @@ typedef enum mcrt_material_type {
 } mcrt_material_type;
 
 enum {
-    MCRT_DEVICE_PTRS = 1 /* buffers are device pointers, call is async    */
+    MCRT_DEVICE_PTRS = 1, /* buffers are device pointers, call is async   */
+    MCRT_BATCH_GRAPH = 2  /* mcrt_batch_create: capture the batch's launch
+                             sequence in a hipGraph on the first render and
+                             replay the graph afterwards                   */
 };
 
 typedef struct mcrt_ctx mcrt_ctx;
 typedef struct mcrt_scene mcrt_scene;
+typedef struct mcrt_batch mcrt_batch;
 
 /* One ray = 8 floats {ox, oy, oz, tmin, dx, dy, dz, tmax} (32 bytes). */
 /* One hit = {prim, t, u, v}; prim = -1 and t = +inf on a miss, as HitInfo
@@ -98,7 +102,9 @@ typedef struct mcrt_render_params {
 MCRT_API const char* mcrt_last_error(void);
 MCRT_API const char* mcrt_version(void);
 
-/* Lifecycle.  `device` is a HIP device ordinal. */
+/* Lifecycle.  `device` is a HIP device ordinal.  The context owns one
+ * path-tracer workspace and traversal spill area per stream it has run on
+ * (shared by its scenes); destroy its scenes and batches before it. */
 MCRT_API int mcrt_create(int device, mcrt_ctx** out);
 MCRT_API int mcrt_destroy(mcrt_ctx* ctx);
 MCRT_API int mcrt_device_name(mcrt_ctx* ctx, char* buf, size_t cap);
@@ -171,6 +177,29 @@ MCRT_API int mcrt_render(mcrt_scene* scene, const mcrt_render_params* params, fl
 MCRT_API int mcrt_render_adjoint(mcrt_scene* scene, const mcrt_render_params* params,
                                  const float* dL_dfilm, float* d_albedo, float* d_emission,
                                  float* film, int flags, void* stream);
+
+/* Batched rendering (no reference counterpart: the reference renders one
+ * scene per process): n finalized scenes of one context, each with its own
+ * camera (film size), parameters and film, rendered together -- the paths
+ * of all scenes share the wavefront chunks and every stage is one launch
+ * for all of them.  Every film equals what mcrt_render gives for that
+ * scene alone, bit for bit.  Traversal tuning and chunk size are those of
+ * scenes[0].  The scenes must stay alive and unmodified (material colours
+ * aside) while the batch exists.
+ *   mcrt_batch_create: films[i] are device pointers (W_i*H_i*3 floats) and
+ *     `flags` must include MCRT_DEVICE_PTRS; MCRT_BATCH_GRAPH captures the
+ *     launch sequence in a hipGraph on the first mcrt_batch_render.
+ *   mcrt_batch_render: renders every film, asynchronously on `stream`.
+ *   mcrt_batch_info: total paths and wavefront chunks of the batch.
+ *   mcrt_render_batch: one-shot create + render + destroy; films are host
+ *     pointers (synchronous) or, with MCRT_DEVICE_PTRS, device pointers. */
+MCRT_API int mcrt_batch_create(size_t n, mcrt_scene* const* scenes, const mcrt_render_params* params,
+                               float* const* films, int flags, mcrt_batch** out);
+MCRT_API int mcrt_batch_render(mcrt_batch* batch, void* stream);
+MCRT_API int mcrt_batch_info(const mcrt_batch* batch, int64_t* total_paths, int64_t* chunks);
+MCRT_API int mcrt_batch_destroy(mcrt_batch* batch);
+MCRT_API int mcrt_render_batch(size_t n, mcrt_scene* const* scenes, const mcrt_render_params* params,
+                               float* const* films, int flags, void* stream);
 
 /* Film output (host memory, W*H*3 floats, row y = 0 at the image bottom);
  * byte-identical to the reference's FrameBuffer writers on finite,

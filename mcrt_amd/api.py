@@ -91,6 +91,11 @@ def lib():
         "mcrt_film_write_hdr": (I, [ctypes.c_char_p, P, ctypes.c_int32, ctypes.c_int32]),
         "mcrt_film_write_ppm": (I, [ctypes.c_char_p, P, ctypes.c_int32, ctypes.c_int32]),
         "mcrt_film_tonemap_gamma": (I, [P, Z, ctypes.c_float]),
+        "mcrt_batch_create": (I, [Z, P, ctypes.POINTER(RenderParams), P, I, ctypes.POINTER(P)]),
+        "mcrt_batch_render": (I, [P, P]),
+        "mcrt_batch_info": (I, [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+        "mcrt_batch_destroy": (I, [P]),
+        "mcrt_render_batch": (I, [Z, P, ctypes.POINTER(RenderParams), P, I, P]),
         "mcrt_scene_set_tuning": (I, [P, ctypes.POINTER(Tuning)]),
         "mcrt_scene_get_tuning": (I, [P, ctypes.POINTER(Tuning)]),
         "mcrt_autotune": (I, [P, P, Z, ctypes.POINTER(RenderParams), I, P, ctypes.POINTER(Tuning),
@@ -329,6 +334,73 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+BATCH_GRAPH = 2
+
+
+def _batch_arrays(scenes, params):
+    n = len(scenes)
+    if len(params) != n:
+        raise ValueError("one parameter dict per scene")
+    hs = (ctypes.c_void_p * n)(*[sc.h.value for sc in scenes])
+    ps = (RenderParams * n)(*[Scene.params(**p) for p in params])
+    return n, hs, ps
+
+
+class Batch:
+    """Batched rendering of several scenes into device films (CUDA tensors,
+    (H_i, W_i, 3) float32) with one launch sequence -- optionally captured
+    in a hipGraph on the first render and replayed after (graph=True)."""
+
+    def __init__(self, scenes, params, films, graph=False):
+        n, hs, ps = _batch_arrays(scenes, params)
+        fp = []
+        for f in films:
+            p, dev = _ptr(f)
+            if not dev:
+                raise ValueError("Batch films must be device tensors")
+            fp.append(p)
+        self._keep = (list(scenes), list(films))
+        h = ctypes.c_void_p()
+        _check(lib().mcrt_batch_create(n, hs, ps, (ctypes.c_void_p * n)(*fp),
+                                       DEVICE_PTRS | (BATCH_GRAPH if graph else 0), ctypes.byref(h)))
+        self.h = h
+
+    def render(self):
+        """Renders every film on the current stream (asynchronous)."""
+        _check(lib().mcrt_batch_render(self.h, _stream_of(True)))
+
+    def info(self):
+        paths, chunks = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().mcrt_batch_info(self.h, ctypes.byref(paths), ctypes.byref(chunks)))
+        return {"paths": paths.value, "chunks": chunks.value}
+
+    def close(self):
+        if self.h:
+            _check(lib().mcrt_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render_batch(scenes, params, films=None):
+    """One-shot batched render; returns the films (host arrays unless
+    device tensors are given)."""
+    n, hs, ps = _batch_arrays(scenes, params)
+    if films is None:
+        films = [np.empty((sc.camera[3], sc.camera[2], 3), np.float32) for sc in scenes]
+    fp = [_ptr(f) for f in films]
+    dev = fp[0][1] if fp else False
+    if any(d != dev for _, d in fp):
+        raise ValueError("films must be all host or all device")
+    _check(lib().mcrt_render_batch(n, hs, ps, (ctypes.c_void_p * n)(*[p for p, _ in fp]),
+                                   DEVICE_PTRS if dev else 0, _stream_of(dev)))
+    return films
 
 
 def _host_film(film):

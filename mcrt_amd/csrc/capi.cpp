@@ -76,6 +76,19 @@ int mcrt::DevBuf::upload(const void* src, size_t n)
     return MCRT_OK;
 }
 
+mcrt_batch::~mcrt_batch()
+{
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap)
+    {
+        (void)hipStreamSynchronize(cap);
+        (void)hipStreamDestroy(cap);
+    }
+    // the workspace may still be in use by a replay enqueued on a user stream
+    (void)hipDeviceSynchronize();
+}
+
 extern "C" {
 
 MCRT_API const char* mcrt_last_error(void) { return g_err.c_str(); }
@@ -108,6 +121,7 @@ MCRT_API int mcrt_create(int device, mcrt_ctx** out)
 MCRT_API int mcrt_destroy(mcrt_ctx* ctx)
 {
     if (!ctx) return fail(MCRT_ERR_INVALID_ARG, "mcrt_destroy: NULL context");
+    (void)hipSetDevice(ctx->device);  // the per-stream workspaces are freed here
     delete ctx;
     return MCRT_OK;
 }
@@ -517,6 +531,111 @@ MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcr
     HIPCHK(hipStreamSynchronize(st));
     if (best) mcrt_scene_get_tuning(s, best);
     if (best_mrays_per_s) *best_mrays_per_s = best_rate;
+    return MCRT_OK;
+}
+
+static int check_batch(size_t n, mcrt_scene* const* scenes, const mcrt_render_params* params,
+                       float* const* films, const char* who)
+{
+    if (n == 0 || !scenes || !params || !films) return fail(MCRT_ERR_INVALID_ARG, "%s: empty batch or NULL array", who);
+    if (n > 65535) return fail(MCRT_ERR_INVALID_ARG, "%s: at most 65535 scenes per batch", who);
+    for (size_t i = 0; i < n; i++)
+    {
+        if (!scenes[i] || !films[i]) return fail(MCRT_ERR_INVALID_ARG, "%s: NULL scene or film %zu", who, i);
+        if (scenes[i]->ctx != scenes[0]->ctx)
+            return fail(MCRT_ERR_INVALID_ARG, "%s: scene %zu belongs to another context", who, i);
+        int rc = check_render(scenes[i], &params[i], who);
+        if (rc) return rc;
+    }
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_batch_create(size_t n, mcrt_scene* const* scenes, const mcrt_render_params* params,
+                               float* const* films, int flags, mcrt_batch** out)
+{
+    if (!out) return fail(MCRT_ERR_INVALID_ARG, "mcrt_batch_create: NULL out");
+    *out = nullptr;
+    int rc = check_batch(n, scenes, params, films, "mcrt_batch_create");
+    if (rc) return rc;
+    if (!(flags & MCRT_DEVICE_PTRS))
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_batch_create: films must be device pointers (MCRT_DEVICE_PTRS)");
+    HIPCHK(hipSetDevice(scenes[0]->ctx->device));
+    std::unique_ptr<mcrt_batch> b(new mcrt_batch());
+    b->ctx = scenes[0]->ctx;
+    b->scenes.assign(scenes, scenes + n);
+    b->params.assign(params, params + n);
+    b->films.assign(films, films + n);
+    b->flags = flags;
+    if ((rc = batch_init(b.get()))) return rc;
+    *out = b.release();
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_batch_render(mcrt_batch* b, void* stream)
+{
+    if (!b) return fail(MCRT_ERR_INVALID_ARG, "mcrt_batch_render: NULL batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> g(b->res.mu);
+    if (!(b->flags & MCRT_BATCH_GRAPH)) return batch_enqueue(b, st);
+    if (!b->exec)
+    {
+        if (!b->cap) HIPCHK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+        HIPCHK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_enqueue(b, b->cap);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->cap, &graph);
+        if (rc) return rc;
+        HIPCHK(e);
+        b->graph = graph;
+        HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(b->exec, st));
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_batch_info(const mcrt_batch* b, int64_t* total_paths, int64_t* chunks)
+{
+    if (!b) return fail(MCRT_ERR_INVALID_ARG, "mcrt_batch_info: NULL batch");
+    if (total_paths) *total_paths = b->total_paths;
+    if (chunks) *chunks = (int64_t)b->chunks.size();
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_batch_destroy(mcrt_batch* b)
+{
+    if (!b) return MCRT_OK;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    delete b;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_render_batch(size_t n, mcrt_scene* const* scenes, const mcrt_render_params* params,
+                               float* const* films, int flags, void* stream)
+{
+    int rc = check_batch(n, scenes, params, films, "mcrt_render_batch");
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(scenes[0]->ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<float*> dev(films, films + n);
+    DevBuf staging;
+    std::vector<size_t> off(n + 1, 0);
+    for (size_t i = 0; i < n; i++)
+        off[i + 1] = off[i] + (size_t)scenes[i]->dev_cam.w * scenes[i]->dev_cam.h * 3;
+    if (!(flags & MCRT_DEVICE_PTRS))
+    {
+        if ((rc = staging.ensure(off[n] * sizeof(float)))) return rc;
+        for (size_t i = 0; i < n; i++) dev[i] = staging.as<float>() + off[i];
+    }
+    mcrt_batch* b = nullptr;
+    if ((rc = mcrt_batch_create(n, scenes, params, dev.data(), MCRT_DEVICE_PTRS, &b))) return rc;
+    std::unique_ptr<mcrt_batch> hold(b);
+    if ((rc = batch_enqueue(b, st))) return rc;
+    if (!(flags & MCRT_DEVICE_PTRS))
+        for (size_t i = 0; i < n; i++)
+            HIPCHK(hipMemcpyAsync(films[i], dev[i], (off[i + 1] - off[i]) * sizeof(float), hipMemcpyDeviceToHost, st));
+    // the batch's workspace is freed on return: wait for its kernels
+    HIPCHK(hipStreamSynchronize(st));
     return MCRT_OK;
 }
 

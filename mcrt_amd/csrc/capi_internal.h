@@ -12,13 +12,6 @@
 #include "geom.h"
 #include "mcrt.h"
 
-struct mcrt_ctx
-{
-    int device = 0;
-    int num_cus = 0;
-    std::string name;
-};
-
 namespace mcrt
 {
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -73,6 +66,7 @@ struct PTWork
     DevBuf T, L, flags, rays[2], act[2], rec[2], cnt, hits, samples, film;
     DevBuf Ltot, delta;                    // adjoint
     DevBuf dfilm, galb, gem;               // adjoint staging for host pointers
+    DevBuf sid, pkey, skey, rsid[2];       // batched rendering
 };
 
 // Device buffers of one adjoint render.
@@ -83,11 +77,11 @@ struct AdjointIO
     float* d_emission;   // ntri*3
 };
 
-// Device resources of one stream on one scene.  Work enqueued on distinct
+// Device resources of one stream in a context.  Work enqueued on distinct
 // streams may overlap on the GPU, so each stream has its own traversal
-// spill stacks and path-tracer workspace; `mu` serialises host threads
-// enqueueing on the same stream (a render's kernels must not interleave
-// with another's on one stream).
+// spill stacks and path-tracer workspace (grown to the largest scene that
+// used it); `mu` serialises host threads enqueueing on the same stream (a
+// render's kernels must not interleave with another's on one stream).
 struct StreamRes
 {
     std::mutex mu;
@@ -96,6 +90,17 @@ struct StreamRes
     int64_t pt_capacity = 0;
 };
 }  // namespace mcrt
+
+struct mcrt_ctx
+{
+    int device = 0;
+    int num_cus = 0;
+    std::string name;
+    // per-stream spill stacks and path-tracer workspaces, shared by the
+    // scenes of the context (work on one stream runs in order)
+    std::mutex res_mu;
+    std::map<hipStream_t, std::unique_ptr<mcrt::StreamRes>> res;
+};
 
 struct mcrt_scene
 {
@@ -118,9 +123,6 @@ struct mcrt_scene
     // staging for host-pointer queries
     std::mutex q_mu;
     mcrt::DevBuf q_in, q_out;
-    // per-stream spill stacks and path-tracer workspaces
-    std::mutex res_mu;
-    std::map<hipStream_t, std::unique_ptr<mcrt::StreamRes>> res;
     // launch tuning (mcrt_scene_set_tuning / mcrt_autotune)
     // defaults: the fastest closest-hit setting on MI355X for the 1.3M-triangle
     // displaced-icosphere scene (24 x 6 KB LDS and 24 x 68 VGPRs fit one CU)
@@ -136,16 +138,53 @@ int finalize_scene(mcrt_scene* s, int threads);
 int upload_materials(mcrt_scene* s);
 int run_query(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, uint8_t* occ, int mode, int flags,
               hipStream_t stream, uint64_t* counters);
-// The resources of `stream` on `s` (created on first use; live as long as s).
+// The resources of `stream` in s's context (created on first use; live as
+// long as the context).
 StreamRes* stream_res(mcrt_scene* s, hipStream_t stream);
 // Sets the traversal tuning of s->dev (LDS depth, spill geometry).
 void apply_trace_tuning(mcrt_scene* s);
 // The scene's device view for a launch with r's spill stacks (caller holds r->mu).
 int launch_view(mcrt_scene* s, StreamRes* r, DevScene& view);
-int pt_workspace(StreamRes* r, int64_t P, bool adjoint);
+int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch);
 // Renders into a device film (W*H*3 floats, may be NULL with adj) on
 // `stream` (asynchronous); with adj also the path-replay gradients.  The
 // caller holds r->mu.
 int render_scene(mcrt_scene* s, StreamRes* r, const mcrt_render_params* p, float* film, const AdjointIO* adj,
                  hipStream_t stream);
+
+// One wavefront chunk of a batched render: P paths over nseg segments
+// starting at segment seg0, npix pixels.
+struct BatchChunk
+{
+    int64_t P, nseg, seg0, npix;
+};
+}  // namespace mcrt
+
+// A batched render: scenes, parameters and device films fixed at creation;
+// device tables, chunking, workspace and (optionally) the captured graph.
+struct mcrt_batch
+{
+    mcrt_ctx* ctx = nullptr;
+    std::vector<mcrt_scene*> scenes;
+    std::vector<mcrt_render_params> params;
+    std::vector<float*> films;
+    int flags = 0;
+    int max_depth = 0;
+    int64_t total_paths = 0;
+    std::vector<mcrt::BatchChunk> chunks;
+    mcrt::DevBuf d_tab, d_roots, d_seg;
+    mcrt::StreamRes res;
+    mcrt::DevScene trace_view;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~mcrt_batch();
+};
+
+namespace mcrt
+{
+// Builds the device tables, chunks and workspace of a validated batch.
+int batch_init(mcrt_batch* b);
+// Enqueues every kernel of the batch on `stream` (launches only).
+int batch_enqueue(mcrt_batch* b, hipStream_t stream);
 }  // namespace mcrt

@@ -51,6 +51,13 @@ struct DevScene
     int32_t lds_depth = 16;            // LDS stack entries per lane (8, 12, 16 or 24)
 };
 
+// The part of a scene a traversal reads (batched launches look it up per ray).
+struct SceneRoot
+{
+    const WNode* nodes = nullptr;
+    const WRef* refs = nullptr;
+};
+
 // Traversal modes.
 enum TraceMode
 {

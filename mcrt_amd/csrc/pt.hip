@@ -1,9 +1,9 @@
-// Wavefront path tracer (mcrt_render) and its path-replay adjoint
-// (mcrt_render_adjoint): the reference's rayTrace integrator
+// Wavefront path tracer (mcrt_render, mcrt_batch_*) and its path-replay
+// adjoint (mcrt_render_adjoint): the reference's rayTrace integrator
 // (demo_pt.cpp:512-655) and render loop (demo_pt.cpp:657-720) as a
 // sequence of kernels over SoA path state.
 //
-// Per chunk of P = pixels x spp paths (path p = pixel-major, sample-minor):
+// Per chunk of P paths (path p = pixel-major, sample-minor):
 //   generate             camera rays with the tent filter
 //   for k = 0 .. max_depth + 1:
 //     extend             one traversal launch over the compacted
@@ -19,6 +19,15 @@
 // Every kernel is grid-strided over a fixed grid and reads its work count
 // from device memory, so the whole chunk is launch-only (graph-capturable).
 //
+// Batched rendering (BATCH = true): the paths of many scenes -- each with
+// its own BVH, camera, film size, spp and depth -- share the chunks.  A
+// chunk is a list of segments (runs of whole pixels of one scene); each
+// path keeps its scene id and its random-number key, each ray its scene
+// id, and the kernels read the scene's data from a device table, so one
+// launch per stage serves every scene of the chunk and the uneven scenes
+// fill the machine together.  The numbers of every path are those of the
+// same scene rendered alone.
+//
 // Adjoint (path replay, Vicini, Speierer, Jakob, "Path Replay
 // Backpropagation", SIGGRAPH 2021): sampling, MIS weights and the roulette
 // (fixed survival probability) do not depend on albedo or emission, so
@@ -30,6 +39,7 @@
 // delta * (coefficient of Le) to the emitting triangle of every emission
 // term.  Memory is independent of path length.
 #include <algorithm>
+#include <vector>
 
 #include "capi_internal.h"
 #include "pt_core.h"
@@ -56,18 +66,44 @@ struct PTState
     float4* delta;       // adjoint: dLoss/dL per path
     float* g_alb;        // adjoint: ntri x 3
     float* g_em;         // adjoint: ntri x 3
+    uint16_t* sid;       // batch: scene of each path
+    uint64_t* pkey;      // batch: mix64(mix64(seed) ^ pixel) of each path
+    uint64_t* skey;      // batch: sample << 8 of each path
+    uint16_t* rsid[2];   // batch: scene of each ray slot (3P)
+};
+
+// One scene of a render: its device data, camera and parameters.
+struct ScenePT
+{
+    DevScene D;
+    DevCamera cam;
+    int32_t spp, max_depth, rr_start, nee;
+    float rr_prob;
+    float scale;         // (float)(1.0 / spp), FrameBuffer::scale (film.cpp:254-260)
+    uint64_t seed;
+    int64_t sample_offset;
+    float* film;         // batch: this scene's device film
+};
+
+// A run of whole pixels of one scene inside a batched chunk.
+struct Segment
+{
+    int32_t sid, spp;
+    int64_t pix0;        // first pixel (index in the scene's film)
+    int64_t p0;          // first chunk-local path
+    int64_t q0;          // first chunk-local pixel
 };
 
 struct PTArgs
 {
-    int64_t P;           // paths in this chunk
-    int64_t pix0;        // first pixel of the chunk
-    int32_t spp;
-    int32_t max_depth, rr_start, nee;
-    float rr_prob;
-    uint64_t seed;
-    int64_t sample_offset;
-    DevCamera cam;
+    int64_t P;              // paths in this chunk
+    int64_t pix0;           // single scene: first pixel of the chunk
+    int32_t nseg;           // batch: segments of this chunk
+    int32_t pad;
+    const Segment* seg;     // batch
+    const ScenePT* tab;     // batch: per-scene table
+    const SceneRoot* roots; // batch: per-scene traversal roots
+    ScenePT one;            // single scene
 };
 
 __device__ __forceinline__ void store_ray(float4* rays, int64_t i, V3f o, V3f d, float tmin, float tmax)
@@ -171,30 +207,67 @@ __global__ void k_fill(float* p, int64_t n, float v)
         p[i] = v;
 }
 
+
+// Last segment whose first path (first pixel with Q) is <= x.
+template <bool Q>
+__device__ __forceinline__ int find_seg(const Segment* seg, int nseg, int64_t x)
+{
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((Q ? seg[mid].q0 : seg[mid].p0) <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // Camera::makeRay (camera.cpp:37-43) + tent filter (demo_pt.cpp:687-699).
+template <bool BATCH>
 __global__ void k_generate(PTState S, PTArgs a)
 {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.P; p += (int64_t)gridDim.x * blockDim.x)
     {
-        const int64_t pixel = a.pix0 + p / a.spp;
-        const uint64_t sample = (uint64_t)(a.sample_offset + p % a.spp);
-        const int x = (int)(pixel % a.cam.w), y = (int)(pixel / a.cam.w);
-        float dx = 0.0f, dy = 0.0f;
-        if (a.spp > 1)
+        const ScenePT* sp = &a.one;
+        int64_t pixel;
+        uint64_t sample;
+        if (BATCH)
         {
-            const uint64_t key = rng_key(a.seed, (uint64_t)pixel, sample, 255u);
+            const Segment g = a.seg[find_seg<false>(a.seg, a.nseg, p)];
+            sp = &a.tab[g.sid];
+            const int64_t lp = p - g.p0;
+            pixel = g.pix0 + lp / g.spp;
+            sample = (uint64_t)(sp->sample_offset + lp % g.spp);
+            S.sid[p] = (uint16_t)g.sid;
+            S.rsid[0][p] = (uint16_t)g.sid;
+            S.pkey[p] = mix64(mix64(sp->seed) ^ (uint64_t)pixel);
+            S.skey[p] = sample << 8;
+        }
+        else
+        {
+            pixel = a.pix0 + p / a.one.spp;
+            sample = (uint64_t)(a.one.sample_offset + p % a.one.spp);
+        }
+        const DevCamera& cam = sp->cam;
+        const int x = (int)(pixel % cam.w), y = (int)(pixel / cam.w);
+        float dx = 0.0f, dy = 0.0f;
+        if (sp->spp > 1)
+        {
+            const uint64_t key = rng_key(sp->seed, (uint64_t)pixel, sample, 255u);
             float r1 = 2.0f * rng_u01(key, 0);
             float r2 = 2.0f * rng_u01(key, 1);
             dx = r1 < 1 ? sqrtf(r1) - 1.0f : 1.0f - sqrtf(2.0f - r1);
             dy = r2 < 1 ? sqrtf(r2) - 1.0f : 1.0f - sqrtf(2.0f - r2);
         }
-        float mx = (((float)x + 0.5f) + dx) / (float)a.cam.w;
-        float my = (((float)y + 0.5f) + dy) / (float)a.cam.h;
-        V3f su = (ld3(a.cam.fx) * (mx - 0.5f)) * a.cam.ww;
-        V3f sv = (ld3(a.cam.fy) * (my - 0.5f)) * 1.0f;
-        V3f sw = ld3(a.cam.fwd) * a.cam.dist;
+        float mx = (((float)x + 0.5f) + dx) / (float)cam.w;
+        float my = (((float)y + 0.5f) + dy) / (float)cam.h;
+        V3f su = (ld3(cam.fx) * (mx - 0.5f)) * cam.ww;
+        V3f sv = (ld3(cam.fy) * (my - 0.5f)) * 1.0f;
+        V3f sw = ld3(cam.fwd) * cam.dist;
         V3f d = nrm((su + sv) + sw);
-        store_ray(S.rays[0], p, ld3(a.cam.o), d, kEpsRay, __builtin_inff());
+        store_ray(S.rays[0], p, ld3(cam.o), d, kEpsRay, __builtin_inff());
         S.act[0][p] = (uint32_t)p;
         S.T[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         S.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -204,11 +277,12 @@ __global__ void k_generate(PTState S, PTArgs a)
 
 // Next-event contributions of one diffuse vertex (demo_pt.cpp:591-643 with
 // Scene::lightPdfW scene.cpp:239-267 and lightRadiance :345-366).
-template <bool ADJ>
-__global__ void k_resolve(DevScene D, PTState S, int cur, int64_t P)
+template <bool ADJ, bool BATCH>
+__global__ void k_resolve(PTArgs a, PTState S, int cur)
 {
     __shared__ float acc[ADJ ? 2 * kHotFloats : 1];
     if (ADJ) hot_clear(acc);
+    const int64_t P = a.P;
     const uint32_t nrec = S.cnt[cur][1] / 2;
     const float4* R = S.rec[cur];
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nrec; base += (int64_t)gridDim.x * blockDim.x)
@@ -224,6 +298,7 @@ __global__ void k_resolve(DevScene D, PTState S, int cur, int64_t P)
             const float pdf_b = r0.w, factor2 = r1.w;
             const uint32_t path = bits_of(r2.w);
             const int li = (int)bits_of(r3.w);
+            const DevScene& D = BATCH ? a.tab[S.sid[path]].D : a.one.D;
             const int4 h1 = S.hits[P + 2 * m];
             V3f local = v3f(0.0f, 0.0f, 0.0f);
             float pdf_l = 0.0f;
@@ -292,12 +367,12 @@ __global__ void k_resolve(DevScene D, PTState S, int cur, int64_t P)
             grad_add(acc + kHotFloats, S.g_em, em2, hot2, g2, em2 >= 0);
         }
     }
-    if (ADJ) hot_flush(D, acc, S.g_alb, S.g_em);
+    if (ADJ) hot_flush(a.one.D, acc, S.g_alb, S.g_em);
 }
 
 // One vertex of every live path (demo_pt.cpp:523-651).
-template <bool ADJ>
-__global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
+template <bool ADJ, bool BATCH>
+__global__ void k_shade(PTArgs a, PTState S, int cur, int k)
 {
     __shared__ float acc[ADJ ? 2 * kHotFloats : 1];
     if (ADJ) hot_clear(acc);
@@ -315,6 +390,9 @@ __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
             const uint32_t p = S.act[cur][j];
             const int4 h = S.hits[j];
             if (h.x < 0) break;  // miss: no environment map
+            const uint16_t sid = BATCH ? S.sid[p] : 0;
+            const ScenePT& sp = BATCH ? a.tab[sid] : a.one;
+            const DevScene& D = sp.D;
             const float4 ra = rays[2 * j], rb = rays[2 * j + 1];
             const V3f o = f3(ra), d = f3(rb);
             const float t = float_of((uint32_t)h.y), u = float_of((uint32_t)h.z), v = float_of((uint32_t)h.w);
@@ -323,7 +401,7 @@ __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
             const V3f col = ld3(mt.rgb);
             const V3f phit = o + d * t;
             const V3f ns = shading_normal(ts, u, v);
-            const bool no_direct = !a.nee || k == 0 || !S.last_diff[p];
+            const bool no_direct = !sp.nee || k == 0 || !S.last_diff[p];
             V3f T = f3(S.T[p]);
             if (mt.type == MCRT_LGHT)
             {
@@ -352,10 +430,16 @@ __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
                             col.z > 0.0f ? dl.z * rest.z / col.z : 0.0f);
             }
             S.last_diff[p] = mt.type == MCRT_DIFF;
-            const int64_t pixel = a.pix0 + p / a.spp;
-            const uint64_t sample = (uint64_t)(a.sample_offset + p % a.spp);
-            const uint64_t key = rng_key(a.seed, (uint64_t)pixel, sample, (uint32_t)k);
-            if (a.nee && mt.type == MCRT_DIFF && D.nlights > 0)
+            uint64_t key;
+            if (BATCH)
+                key = mix64(S.pkey[p] ^ (S.skey[p] | (uint64_t)(uint32_t)k));
+            else
+            {
+                const int64_t pixel = a.pix0 + p / sp.spp;
+                const uint64_t sample = (uint64_t)(sp.sample_offset + p % sp.spp);
+                key = rng_key(sp.seed, (uint64_t)pixel, sample, (uint32_t)k);
+            }
+            if (sp.nee && mt.type == MCRT_DIFF && D.nlights > 0)
             {
                 const V3f wb = lambert_dir(ns, rng_u01(key, 0), rng_u01(key, 1));
                 const float pdf_b = fmax_ref(dot(wb, ns), kEpsCos) / kPi;
@@ -398,6 +482,11 @@ __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
                     store_ray(nr, a.P + 2 * m + 1, phit, wo, kEpsRay, __builtin_inff());
                 else  // zero direction: the slab test rejects every box (ray.cpp:77-79)
                     store_ray(nr, a.P + 2 * m + 1, phit, v3f(0.0f, 0.0f, 0.0f), kEpsRay, __builtin_inff());
+                if (BATCH)
+                {
+                    S.rsid[nxt][a.P + 2 * m] = sid;
+                    S.rsid[nxt][a.P + 2 * m + 1] = sid;
+                }
             }
             // Continuation (Material::sample, demo_pt.cpp:645-650)
             const V3f wi = -d;
@@ -414,16 +503,17 @@ __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
                 weight = col;
             }
             T = T * weight;
-            if (k >= a.max_depth) break;
-            if (k >= a.rr_start)
+            if (k >= sp.max_depth) break;
+            if (k >= sp.rr_start)
             {
-                if (rng_u01(key, 7) >= a.rr_prob) break;
-                T = T * (1.0f / a.rr_prob);
+                if (rng_u01(key, 7) >= sp.rr_prob) break;
+                T = T * (1.0f / sp.rr_prob);
             }
             S.T[p] = f4(T);
             const uint32_t slot = atomicAdd(&S.cnt[nxt][0], 1u);
             store_ray(S.rays[nxt], slot, phit, nd, kEpsRay, __builtin_inff());
             S.act[nxt][slot] = p;
+            if (BATCH) S.rsid[nxt][slot] = sid;
         } while (false);
         if (ADJ)
         {
@@ -431,7 +521,7 @@ __global__ void k_shade(DevScene D, PTState S, PTArgs a, int cur, int k)
             grad_add(acc + kHotFloats, S.g_em, em_key, hot, g_em, em_key >= 0);
         }
     }
-    if (ADJ) hot_flush(D, acc, S.g_alb, S.g_em);
+    if (ADJ) hot_flush(a.one.D, acc, S.g_alb, S.g_em);
 }
 
 // demo_pt.cpp:653 (NaN -> 0) and :703 (clamp >= 0 per channel).
@@ -455,7 +545,7 @@ __global__ void k_delta(PTState S, PTArgs a, const float* dfilm, float scale)
         const float4 L = S.L[p];
         S.Ltot[p] = L;
         const bool nan = !(L.x == L.x && L.y == L.y && L.z == L.z);
-        const float* g = dfilm + 3 * (a.pix0 + p / a.spp);
+        const float* g = dfilm + 3 * (a.pix0 + p / a.one.spp);
         S.delta[p] = make_float4(!nan && L.x >= 0.0f ? g[0] * scale : 0.0f,
                                  !nan && L.y >= 0.0f ? g[1] * scale : 0.0f,
                                  !nan && L.z >= 0.0f ? g[2] * scale : 0.0f, 0.0f);
@@ -463,20 +553,43 @@ __global__ void k_delta(PTState S, PTArgs a, const float* dfilm, float scale)
 }
 
 // FrameBuffer::accumulatePixel over the samples in order, then
-// scale((float)(1.0/spp)) (demo_pt.cpp:700-710, film.cpp:204-260).
-__global__ void k_accumulate(PTState S, float* film, int64_t pix0, int64_t npix, int spp, float scale)
+// scale((float)(1.0/spp)) (demo_pt.cpp:700-710, film.cpp:204-260), for the
+// npix pixels of the chunk.
+template <bool BATCH>
+__global__ void k_accumulate(PTState S, PTArgs a, float* film, int64_t npix)
 {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < npix; q += (int64_t)gridDim.x * blockDim.x)
     {
+        int spp;
+        int64_t p0, pixel;
+        float scale;
+        float* out;
+        if (BATCH)
+        {
+            const Segment g = a.seg[find_seg<true>(a.seg, a.nseg, q)];
+            spp = g.spp;
+            p0 = g.p0 + (q - g.q0) * spp;
+            pixel = g.pix0 + (q - g.q0);
+            scale = a.tab[g.sid].scale;
+            out = a.tab[g.sid].film;
+        }
+        else
+        {
+            spp = a.one.spp;
+            p0 = q * spp;
+            pixel = a.pix0 + q;
+            scale = a.one.scale;
+            out = film;
+        }
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
         for (int s = 0; s < spp; s++)
         {
-            const float4 v = S.samples[q * spp + s];
+            const float4 v = S.samples[p0 + s];
             gx = gx + v.x;
             gy = gy + v.y;
             gz = gz + v.z;
         }
-        float* f = film + 3 * (pix0 + q);
+        float* f = out + 3 * pixel;
         f[0] = gx * scale;
         f[1] = gy * scale;
         f[2] = gz * scale;
@@ -492,31 +605,80 @@ unsigned grid_for(int64_t n, int num_cus)
     return (unsigned)b;
 }
 
-// The bounce loop of one chunk (generate .. last resolve).
-template <bool ADJ>
-void bounce_loop(const DevScene& D, PTState& S, const PTArgs& a, int max_depth, unsigned g, hipStream_t st)
+// The bounce loop of one chunk (generate .. last resolve).  Dt carries the
+// traversal settings (LDS depth, spill stacks) of the launch.
+template <bool ADJ, bool BATCH>
+void bounce_loop(const DevScene& Dt, PTState& S, const PTArgs& a, int max_depth, unsigned g, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_generate, dim3(g), dim3(kTPB), 0, st, S, a);
+    hipLaunchKernelGGL(k_generate<BATCH>, dim3(g), dim3(kTPB), 0, st, S, a);
     hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[0], (uint32_t)a.P, 0u);
     int cur = 0;
     for (int k = 0; k <= max_depth + 1; k++)
     {
         const int nxt = cur ^ 1;
-        launch_k_trace<false, false, true>(D, trace_blocks(D, 3 * a.P), st, S.rays[cur], (int64_t)0, S.cnt[cur],
-                                           a.P, S.hits, (uint8_t*)nullptr, (unsigned long long*)nullptr);
-        hipLaunchKernelGGL(k_resolve<ADJ>, dim3(g), dim3(kTPB), 0, st, D, S, cur, a.P);
+        TraceBatch tb;
+        tb.roots = a.roots;
+        tb.rsid = S.rsid[cur];
+        launch_k_trace<false, false, true, BATCH>(Dt, trace_blocks(Dt, 3 * a.P), st, S.rays[cur], (int64_t)0,
+                                                  S.cnt[cur], a.P, S.hits, (uint8_t*)nullptr,
+                                                  (unsigned long long*)nullptr, tb);
+        hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(g), dim3(kTPB), 0, st, a, S, cur);
         if (k <= max_depth)
         {
             hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[nxt], 0u, 0u);
-            hipLaunchKernelGGL(k_shade<ADJ>, dim3(g), dim3(kTPB), 0, st, D, S, a, cur, k);
+            hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(g), dim3(kTPB), 0, st, a, S, cur, k);
         }
         cur = nxt;
     }
 }
+
+PTState state_of(StreamRes* r, bool adjoint)
+{
+    PTWork& w = r->ptw;
+    PTState S;
+    S.T = w.T.as<float4>();
+    S.L = w.L.as<float4>();
+    S.last_diff = w.flags.as<uint8_t>();
+    for (int i = 0; i < 2; i++)
+    {
+        S.rays[i] = w.rays[i].as<float4>();
+        S.act[i] = w.act[i].as<uint32_t>();
+        S.rec[i] = w.rec[i].as<float4>();
+        S.cnt[i] = w.cnt.as<uint32_t>() + 2 * i;
+        S.rsid[i] = w.rsid[i].as<uint16_t>();
+    }
+    S.hits = w.hits.as<int4>();
+    S.samples = w.samples.as<float4>();
+    S.Ltot = adjoint ? w.Ltot.as<float4>() : nullptr;
+    S.delta = adjoint ? w.delta.as<float4>() : nullptr;
+    S.g_alb = S.g_em = nullptr;
+    S.sid = w.sid.as<uint16_t>();
+    S.pkey = w.pkey.as<uint64_t>();
+    S.skey = w.skey.as<uint64_t>();
+    return S;
+}
+
+ScenePT scene_pt(mcrt_scene* s, const DevScene& D, const mcrt_render_params* prm)
+{
+    ScenePT sp;
+    sp.D = D;
+    sp.cam = s->dev_cam;
+    sp.spp = prm->spp;
+    sp.max_depth = prm->max_depth;
+    sp.rr_start = prm->rr_start;
+    sp.nee = prm->nee;
+    sp.rr_prob = prm->rr_prob;
+    sp.scale = (float)(1.0 / prm->spp);
+    sp.seed = prm->seed;
+    sp.sample_offset = prm->sample_offset;
+    sp.film = nullptr;
+    return sp;
+}
 }  // namespace
 
-// Workspace for up to P paths.
-int pt_workspace(StreamRes* r, int64_t P, bool adjoint)
+// Workspace for up to P paths (batch: plus the per-path / per-ray scene ids
+// and random-number keys).
+int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch)
 {
     PTWork& w = r->ptw;
     int rc;
@@ -542,6 +704,14 @@ int pt_workspace(StreamRes* r, int64_t P, bool adjoint)
         if ((rc = w.Ltot.ensure(n * 16))) return rc;
         if ((rc = w.delta.ensure(n * 16))) return rc;
     }
+    if (batch)
+    {
+        if ((rc = w.sid.ensure(n * 2))) return rc;
+        if ((rc = w.pkey.ensure(n * 8))) return rc;
+        if ((rc = w.skey.ensure(n * 8))) return rc;
+        for (int i = 0; i < 2; i++)
+            if ((rc = w.rsid[i].ensure(3 * n * 2))) return rc;
+    }
     return MCRT_OK;
 }
 
@@ -557,60 +727,147 @@ int render_scene(mcrt_scene* s, StreamRes* r, const mcrt_render_params* prm, flo
     int64_t pix_per_chunk = s->pt_chunk_paths / spp;
     if (pix_per_chunk < 1) pix_per_chunk = 1;
     if (pix_per_chunk > npix) pix_per_chunk = npix;
-    if ((rc = pt_workspace(r, pix_per_chunk * spp, adj != nullptr))) return rc;
-    PTWork& w = r->ptw;
-    PTState S;
-    S.T = w.T.as<float4>();
-    S.L = w.L.as<float4>();
-    S.last_diff = w.flags.as<uint8_t>();
-    for (int i = 0; i < 2; i++)
-    {
-        S.rays[i] = w.rays[i].as<float4>();
-        S.act[i] = w.act[i].as<uint32_t>();
-        S.rec[i] = w.rec[i].as<float4>();
-        S.cnt[i] = w.cnt.as<uint32_t>() + 2 * i;
-    }
-    S.hits = w.hits.as<int4>();
-    S.samples = w.samples.as<float4>();
-    S.Ltot = adj ? w.Ltot.as<float4>() : nullptr;
-    S.delta = adj ? w.delta.as<float4>() : nullptr;
+    if ((rc = pt_workspace(r, pix_per_chunk * spp, adj != nullptr, false))) return rc;
+    PTState S = state_of(r, adj != nullptr);
     S.g_alb = adj ? adj->d_albedo : nullptr;
     S.g_em = adj ? adj->d_emission : nullptr;
     const int cus = s->ctx->num_cus;
-    const float scale = (float)(1.0 / spp);
     if (adj)
     {
         const int64_t ng = 3 * s->dev.ntri;
         hipLaunchKernelGGL(k_fill, dim3(grid_for(ng, cus)), dim3(kTPB), 0, st, adj->d_albedo, ng, 0.0f);
         hipLaunchKernelGGL(k_fill, dim3(grid_for(ng, cus)), dim3(kTPB), 0, st, adj->d_emission, ng, 0.0f);
     }
+    PTArgs a;
+    a.nseg = 0;
+    a.pad = 0;
+    a.seg = nullptr;
+    a.tab = nullptr;
+    a.roots = nullptr;
+    a.one = scene_pt(s, D, prm);
     for (int64_t pix0 = 0; pix0 < npix; pix0 += pix_per_chunk)
     {
         const int64_t np = std::min(pix_per_chunk, npix - pix0);
-        PTArgs a;
         a.P = np * spp;
         a.pix0 = pix0;
-        a.spp = spp;
-        a.max_depth = prm->max_depth;
-        a.rr_start = prm->rr_start;
-        a.nee = prm->nee;
-        a.rr_prob = prm->rr_prob;
-        a.seed = prm->seed;
-        a.sample_offset = prm->sample_offset;
-        a.cam = cam;
         const unsigned g = grid_for(a.P, cus);
-        bounce_loop<false>(D, S, a, prm->max_depth, g, st);
+        bounce_loop<false, false>(D, S, a, prm->max_depth, g, st);
         if (adj)
         {
-            hipLaunchKernelGGL(k_delta, dim3(g), dim3(kTPB), 0, st, S, a, adj->dfilm, scale);
-            bounce_loop<true>(D, S, a, prm->max_depth, g, st);
+            hipLaunchKernelGGL(k_delta, dim3(g), dim3(kTPB), 0, st, S, a, adj->dfilm, a.one.scale);
+            bounce_loop<true, false>(D, S, a, prm->max_depth, g, st);
         }
         if (film)
         {
             hipLaunchKernelGGL(k_finish, dim3(g), dim3(kTPB), 0, st, S, a.P);
-            hipLaunchKernelGGL(k_accumulate, dim3(grid_for(np, cus)), dim3(kTPB), 0, st, S, film, pix0, np, spp,
-                               scale);
+            hipLaunchKernelGGL(k_accumulate<false>, dim3(grid_for(np, cus)), dim3(kTPB), 0, st, S, a, film, np);
         }
+        HIPCHK(hipGetLastError());
+    }
+    return MCRT_OK;
+}
+
+// Batched rendering: device tables of the scenes, chunk segments cut at
+// pixel boundaries, traversal settings of the first scene with the largest
+// stack bound of the batch.
+int batch_init(mcrt_batch* b)
+{
+    const size_t n = b->scenes.size();
+    mcrt_scene* s0 = b->scenes[0];
+    int64_t pmax = s0->pt_chunk_paths;
+    int max_sb = 0;
+    b->max_depth = 0;
+    std::vector<ScenePT> tab(n);
+    std::vector<SceneRoot> roots(n);
+    for (size_t i = 0; i < n; i++)
+    {
+        mcrt_scene* s = b->scenes[i];
+        tab[i] = scene_pt(s, s->dev, &b->params[i]);
+        tab[i].film = b->films[i];
+        roots[i].nodes = s->dev.nodes;
+        roots[i].refs = s->dev.refs;
+        pmax = std::max<int64_t>(pmax, b->params[i].spp);
+        max_sb = std::max(max_sb, s->dev.stack_bound);
+        b->max_depth = std::max(b->max_depth, b->params[i].max_depth);
+    }
+    // chunks of whole pixels, at most pmax paths each
+    std::vector<Segment> segs;
+    b->chunks.clear();
+    BatchChunk c{0, 0, 0, 0};
+    b->total_paths = 0;
+    for (size_t i = 0; i < n; i++)
+    {
+        const int spp = b->params[i].spp;
+        const int64_t npix = (int64_t)b->scenes[i]->dev_cam.w * b->scenes[i]->dev_cam.h;
+        for (int64_t pix = 0; pix < npix;)
+        {
+            const int64_t room = (pmax - c.P) / spp;
+            if (room == 0)
+            {
+                b->chunks.push_back(c);
+                c = BatchChunk{0, 0, (int64_t)segs.size(), 0};
+                continue;
+            }
+            const int64_t take = std::min(room, npix - pix);
+            Segment g;
+            g.sid = (int32_t)i;
+            g.spp = spp;
+            g.pix0 = pix;
+            g.p0 = c.P;
+            g.q0 = c.npix;
+            segs.push_back(g);
+            c.nseg++;
+            c.P += take * spp;
+            c.npix += take;
+            pix += take;
+        }
+        b->total_paths += npix * spp;
+    }
+    if (c.nseg) b->chunks.push_back(c);
+    int rc;
+    if ((rc = b->d_tab.upload(tab.data(), tab.size() * sizeof(ScenePT)))) return rc;
+    if ((rc = b->d_roots.upload(roots.data(), roots.size() * sizeof(SceneRoot)))) return rc;
+    if ((rc = b->d_seg.upload(segs.data(), segs.size() * sizeof(Segment)))) return rc;
+    int64_t pcap = 0;
+    for (const BatchChunk& k : b->chunks) pcap = std::max(pcap, k.P);
+    if ((rc = pt_workspace(&b->res, pcap, false, true))) return rc;
+    DevScene& Dt = b->trace_view;
+    Dt = s0->dev;
+    Dt.nodes = nullptr;
+    Dt.refs = nullptr;
+    Dt.lds_depth = s0->trace_lds_depth;
+    Dt.spill_lanes = (int64_t)b->ctx->num_cus * s0->trace_waves_per_cu * 64;
+    Dt.spill_depth = max_sb > Dt.lds_depth ? max_sb - Dt.lds_depth : 0;
+    Dt.spill = nullptr;
+    if (Dt.spill_depth > 0)
+    {
+        if ((rc = b->res.spill.ensure((size_t)Dt.spill_lanes * Dt.spill_depth * sizeof(uint64_t)))) return rc;
+        Dt.spill = b->res.spill.as<uint64_t>();
+    }
+    return MCRT_OK;
+}
+
+// Every kernel of the batch on `st` (launches only: capturable).
+int batch_enqueue(mcrt_batch* b, hipStream_t st)
+{
+    PTState S = state_of(&b->res, false);
+    const int cus = b->ctx->num_cus;
+    PTArgs a;
+    a.pix0 = 0;
+    a.pad = 0;
+    a.tab = b->d_tab.as<ScenePT>();
+    a.roots = b->d_roots.as<SceneRoot>();
+    a.one = ScenePT();
+    for (const BatchChunk& c : b->chunks)
+    {
+        a.P = c.P;
+        a.nseg = (int32_t)c.nseg;
+        a.seg = b->d_seg.as<Segment>() + c.seg0;
+        const unsigned g = grid_for(a.P, cus);
+        bounce_loop<false, true>(b->trace_view, S, a, b->max_depth, g, st);
+        hipLaunchKernelGGL(k_finish, dim3(g), dim3(kTPB), 0, st, S, a.P);
+        hipLaunchKernelGGL(k_accumulate<true>, dim3(grid_for(c.npix, cus)), dim3(kTPB), 0, st, S, a,
+                           (float*)nullptr, c.npix);
         HIPCHK(hipGetLastError());
     }
     return MCRT_OK;

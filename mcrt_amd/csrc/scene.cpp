@@ -242,8 +242,8 @@ void apply_trace_tuning(mcrt_scene* s)
 
 StreamRes* stream_res(mcrt_scene* s, hipStream_t stream)
 {
-    std::lock_guard<std::mutex> g(s->res_mu);
-    std::unique_ptr<StreamRes>& r = s->res[stream];
+    std::lock_guard<std::mutex> g(s->ctx->res_mu);
+    std::unique_ptr<StreamRes>& r = s->ctx->res[stream];
     if (!r) r.reset(new StreamRes());
     return r.get();
 }

@@ -178,8 +178,10 @@ struct Tracer
         sp = 0;
     }
 
-    // Visits `cur`; returns true once the traversal has ended.
-    MCRT_HD bool step(const DevScene& S, TraceStack& st, TraceCount& cnt)
+    // Visits `cur`; returns true once the traversal has ended.  Sc is any
+    // type with the scene's `nodes` and `refs` (DevScene, SceneRoot).
+    template <class Sc>
+    MCRT_HD bool step(const Sc& S, TraceStack& st, TraceCount& cnt)
     {
         if (!(cur & MCRT_CHILD_LEAF))
         {

@@ -8,6 +8,8 @@
 //     path tracer's compacted continuation rays [0, cnt[0]) and its
 //     next-event rays [seg1, seg1 + cnt[1])), so the bounce loop needs no
 //     host round trip and can be captured in a graph.
+// BATCH = true: the rays belong to several scenes (batched rendering); ray
+// i is traversed in the BVH of scene rsid[i].
 #pragma once
 #include "trace_core.h"
 
@@ -15,11 +17,18 @@ namespace mcrt
 {
 constexpr int kWave = 64;
 
-template <int D, bool ANY, bool COUNT, bool SEG>
+// Rays of several scenes in one launch: ray i belongs to scene rsid[i].
+struct TraceBatch
+{
+    const SceneRoot* roots = nullptr;
+    const uint16_t* rsid = nullptr;
+};
+
+template <int D, bool ANY, bool COUNT, bool SEG, bool BATCH>
 __global__ __launch_bounds__(kWave) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
                                                  const uint32_t* __restrict__ seg_cnt, int64_t seg1,
                                                  int4* __restrict__ hits, uint8_t* __restrict__ occ,
-                                                 unsigned long long* __restrict__ counters)
+                                                 unsigned long long* __restrict__ counters, TraceBatch B)
 {
     __shared__ uint32_t s_node[D * kWave];
     __shared__ float s_lo[D * kWave];
@@ -48,17 +57,21 @@ __global__ __launch_bounds__(kWave) void k_trace(DevScene S, const float4* __res
     // starts the next one in the same iteration in which the previous ends,
     // so divergent ray lengths do not idle the wavefront.
     Tracer<ANY, COUNT> tr;
+    SceneRoot root;
+    root.nodes = S.nodes;
+    root.refs = S.refs;
     uint32_t i = 0;
     bool live = k < total;
     if (live)
     {
         i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : k;
+        if (BATCH) root = B.roots[B.rsid[i]];
         const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
         tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w);
     }
     while (live)
     {
-        if (S.nodes != nullptr && !tr.step(S, st, cnt)) continue;
+        if (root.nodes != nullptr && !tr.step(root, st, cnt)) continue;
         float t, u, v;
         const int prim = tr.result(t, u, v);
         if (ANY)
@@ -71,6 +84,7 @@ __global__ __launch_bounds__(kWave) void k_trace(DevScene S, const float4* __res
         if (live)
         {
             i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : k;
+            if (BATCH) root = B.roots[B.rsid[i]];
             const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
             tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w);
         }
@@ -96,28 +110,29 @@ inline int64_t trace_blocks(const DevScene& s, int64_t n)
 }
 
 // Launches k_trace with the scene's LDS stack depth on `blocks` workgroups.
-template <bool ANY, bool COUNT, bool SEG>
+template <bool ANY, bool COUNT, bool SEG, bool BATCH = false>
 inline void launch_k_trace(const DevScene& S, int64_t blocks, hipStream_t st, const float4* rays, int64_t n,
                            const uint32_t* seg_cnt, int64_t seg1, int4* hits, uint8_t* occ,
-                           unsigned long long* counters)
+                           unsigned long long* counters, TraceBatch B = TraceBatch())
 {
     const dim3 g((unsigned)blocks), b(kWave);
     switch (S.lds_depth)
     {
     case 8:
-        hipLaunchKernelGGL((k_trace<8, ANY, COUNT, SEG>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ, counters);
+        hipLaunchKernelGGL((k_trace<8, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+                           counters, B);
         break;
     case 12:
-        hipLaunchKernelGGL((k_trace<12, ANY, COUNT, SEG>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
-                           counters);
+        hipLaunchKernelGGL((k_trace<12, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+                           counters, B);
         break;
     case 24:
-        hipLaunchKernelGGL((k_trace<24, ANY, COUNT, SEG>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
-                           counters);
+        hipLaunchKernelGGL((k_trace<24, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+                           counters, B);
         break;
     default:
-        hipLaunchKernelGGL((k_trace<16, ANY, COUNT, SEG>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
-                           counters);
+        hipLaunchKernelGGL((k_trace<16, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+                           counters, B);
         break;
     }
 }

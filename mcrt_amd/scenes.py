@@ -85,32 +85,36 @@ def icosphere(sub):
 def geodesic(freq):
     """Unit geodesic sphere of frequency `freq` (every icosahedron face cut
     into freq^2 triangles on a barycentric grid, projected to the sphere):
-    20*freq^2 faces, shared vertices, outward winding."""
+    20*freq^2 faces, shared vertices, outward winding.  Per icosahedron
+    face the grid points (i, j), j <= freq - i, are A + (B-A) i/f + (C-A) j/f
+    in i-major order; per cell (i, j) the triangle ((i,j), (i+1,j), (i,j+1))
+    and, inside the face, ((i+1,j), (i+1,j+1), (i,j+1))."""
     iv, ifc = icosphere(0)
+    f = freq
+    ii = np.concatenate([np.full(f + 1 - i, i) for i in range(f + 1)])
+    jj = np.concatenate([np.arange(f + 1 - i) for i in range(f + 1)])
+    fi = np.concatenate([np.full(f - i, i) for i in range(f)])
+    fj = np.concatenate([np.arange(f - i) for i in range(f)])
+
+    def idx(i, j):  # position of grid point (i, j) in a face's point list
+        return i * (f + 1) - (i * (i - 1)) // 2 + j
+
+    t1 = np.stack([idx(fi, fj), idx(fi + 1, fj), idx(fi, fj + 1)], 1)
+    t2 = np.stack([idx(fi + 1, fj), idx(fi + 1, fj + 1), idx(fi, fj + 1)], 1)
+    keep = np.stack([np.ones(fi.size, bool), fi + fj < f - 1], 1).reshape(-1)
+    local = np.stack([t1, t2], 1).reshape(-1, 3)[keep]
     verts, faces = [], []
-    base = 0
-    for a, b, c in ifc:
+    for k, (a, b, c) in enumerate(ifc):
         A, B, C = iv[a], iv[b], iv[c]
-        idx = {}
-        pts = []
-        for i in range(freq + 1):
-            for j in range(freq + 1 - i):
-                idx[(i, j)] = base + len(pts)
-                pts.append(A + (B - A) * (i / freq) + (C - A) * (j / freq))
-        for i in range(freq):
-            for j in range(freq - i):
-                faces.append((idx[(i, j)], idx[(i + 1, j)], idx[(i, j + 1)]))
-                if i + j < freq - 1:
-                    faces.append((idx[(i + 1, j)], idx[(i + 1, j + 1)], idx[(i, j + 1)]))
-        verts.extend(pts)
-        base += len(pts)
-    v = np.array(verts)
+        verts.append((A[None, :] + (B - A)[None, :] * (ii / f)[:, None]) + (C - A)[None, :] * (jj / f)[:, None])
+        faces.append(local + k * ii.size)
+    v = np.concatenate(verts)
     v /= np.linalg.norm(v, axis=1, keepdims=True)
-    f = np.array(faces, np.int64)
+    fc = np.concatenate(faces).astype(np.int64)
     # merge the copies of edge vertices shared by adjacent icosahedron faces
     key = np.round(v * 1e9).astype(np.int64)
     _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
-    return v[first], inv.reshape(-1)[f]
+    return v[first], inv.reshape(-1)[fc]
 
 
 def value_noise3(p, octaves=3, seed=7, lattice=32, base_freq=2.0):
@@ -261,6 +265,30 @@ def inverse_rendering(freq=71, hidden_seed=5):
     init_rgb[0] = 10.0
     init_rgb[4:] = 0.5
     return target, init_rgb
+
+
+C4_SIDES = (128, 256, 384, 512)
+C4_SPP = (8, 16, 32, 64)
+C4_RENDER = dict(max_depth=6, rr_start=3, rr_prob=0.8, seed=1)
+
+
+def c4_batch(n=64, seed=21):
+    """Batched uneven rendering: n scenes of the Cornell box + light around a
+    displaced geodesic sphere (the at-scale recipe: per-triangle albedo,
+    10% Phong 64) with a triangle count log-uniform in [1k, 500k] (frequency
+    round(sqrt(N/20))), square film side uniform in {128, 256, 384, 512}
+    and spp uniform in {8, 16, 32, 64} (seed 21).  Returns a list of
+    (SceneData, side, spp)."""
+    rng = np.random.default_rng(seed)
+    target = np.exp(rng.uniform(np.log(1e3), np.log(5e5), n))
+    sides = rng.choice(C4_SIDES, n)
+    spps = rng.choice(C4_SPP, n)
+    out = []
+    for k in range(n):
+        freq = max(1, int(round(np.sqrt(target[k] / 20.0))))
+        d = cornell_displaced(geodesic_freq=freq, noise_seed=7, albedo_seed=11 + k)
+        out.append((d, int(sides[k]), int(spps[k])))
+    return out
 
 
 def soup(n=1 << 22, seed=3, lmin=0.002, lmax=0.01):
