@@ -201,6 +201,59 @@ def c3_leg(ctx, steps):
     return out
 
 
+def c4_leg(ctx, steps):
+    """Batched uneven rendering: 64 Cornell + displaced-sphere scenes
+    (1k..500k triangles, film side 128..512, 8..64 spp, depth 6, seed 21),
+    rendered plainly (one mcrt_render per scene, in order) and batched (one
+    hipGraph replay of the whole batch); Msamples/s of each and whether the
+    64 films of the two forms are bit-identical."""
+    import torch
+    import mcrt_amd
+    from mcrt_amd import scenes
+    t0 = time.time()
+    batch = scenes.c4_batch()
+    sc, films, params = [], [], []
+    for d, side, spp in batch:
+        sc.append(d.to_scene(ctx, width=side, height=side))
+        films.append(torch.zeros((side, side, 3), dtype=torch.float32, device="cuda"))
+        params.append(dict(scenes.C4_RENDER, spp=spp))
+    log(f"C4: {len(sc)} scenes, {sum(d.ntri for d, _, _ in batch)} triangles, built in {time.time() - t0:.1f} s")
+    nsamp = sum(side * side * spp for _, side, spp in batch)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(torch.cuda.current_stream())
+        for _ in range(steps):
+            fn()
+        e1.record(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def plain():
+        for s, f, p in zip(sc, films, params):
+            s.render(f, **p)
+
+    t_plain = timed(plain)
+    ref = [f.clone() for f in films]
+    for f in films:
+        f.fill_(-1.0)
+    b = mcrt_amd.Batch(sc, params, films, graph=True)
+    t_batch = timed(b.render)
+    same = all(bool(torch.equal(a.view(torch.int32), r.view(torch.int32))) for a, r in zip(films, ref))
+    info = b.info()
+    b.close()
+    for s in sc:
+        s.close()
+    return {"scenes": len(sc), "samples": nsamp, "paths_per_frame": info["paths"], "chunks": info["chunks"],
+            "plain_ms": round(t_plain, 2), "plain_msamples_per_s": round(nsamp / t_plain / 1e3, 2),
+            "batched_ms": round(t_batch, 2), "batched_msamples_per_s": round(nsamp / t_batch / 1e3, 2),
+            "batched_over_plain": round(t_plain / t_batch, 3), "films_bit_identical": same,
+            "config": {"triangles": int(sum(d.ntri for d, _, _ in batch)), "depth": scenes.C4_RENDER["max_depth"],
+                       "seed": 21, "graph": True}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +264,7 @@ def main():
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the 4M-triangle soup ray-query leg")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 64-scene batched rendering leg")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -314,6 +368,8 @@ def main():
         out["msamples_per_s"] = r["msamples_per_s"]
     if not (args.no_c3 or args.profile):
         out["c3_soup"] = c3_leg(ctx, 5)
+    if not (args.no_c4 or args.profile):
+        out["c4_batch"] = c4_leg(ctx, 2)
     if not args.no_adjoint:
         a = adjoint_leg(ctx, 1 if args.profile else 3)
         out["adjoint"] = a
