@@ -218,16 +218,19 @@ struct Tracer
         }
         else
         {
+            // One leaf reference per step, fetched whole (clipped box and
+            // triangle, 64 B, one memory round trip): a wavefront whose lanes
+            // sit in different leaves never serialises several dependent
+            // fetches in one step.  The leaf's remaining references stay in
+            // `cur` and are tested in slot order by the following steps.
             const uint32_t first = cur & MCRT_LEAF_REF_MASK;
             const uint32_t count = ((cur >> 29) & 3u) + 1u;
-            for (uint32_t j = 0; j < count; ++j)
+            const float4* q = reinterpret_cast<const float4*>(S.refs + first);
+            const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+            if (COUNT) cnt.refs++;
+            float lo;
+            if (slab(r, a.x, a.w, a.y, b.x, a.z, b.y, R1, lo))
             {
-                const float4* q = reinterpret_cast<const float4*>(S.refs + first + j);
-                const float4 a = q[0], b = q[1];
-                if (COUNT) cnt.refs++;
-                float lo;
-                if (!slab(r, a.x, a.w, a.y, b.x, a.z, b.y, R1, lo)) continue;
-                const float4 c = q[2], d = q[3];
                 float t, u, v;
                 if (COUNT) cnt.tris++;
                 if (mt(r, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, t, u, v) && t > r.R0 && t < R1)
@@ -238,6 +241,11 @@ struct Tracer
                     prim = (int)bits_of(d.w);
                     if (ANY) return true;
                 }
+            }
+            if (count > 1)
+            {
+                cur = MCRT_CHILD_LEAF | ((count - 2u) << 29) | (first + 1u);
+                return false;
             }
         }
         while (sp > 0)
