@@ -23,7 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-NODE_B, REF_B, TRI_B, RAY_B, HIT_B = 128, 32, 32, 32, 16
+NODE_B, REF_B, TRI_B, RAY_B, HIT_B = 64, 32, 32, 32, 16  # QNode, WRef box half, WRef triangle half
 BOX_LO = np.array([0.5, 0.5, 0.5])
 BOX_HI = np.array([55.1, 54.3, 55.4])
 

@@ -406,8 +406,8 @@ MCRT_API int mcrt_query_counters(mcrt_scene* s, const float* rays, size_t n, int
 
 static int check_tuning(const mcrt_tuning* t)
 {
-    if (t->trace_waves_per_cu < 0 || t->trace_waves_per_cu > 64)
-        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_waves_per_cu in [1, 64] (0 = keep)");
+    if (t->trace_waves_per_cu < 0 || t->trace_waves_per_cu > 1024)
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_waves_per_cu in [1, 1024] (0 = keep)");
     if (t->trace_lds_depth != 0 && !valid_lds_depth(t->trace_lds_depth))
         return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_lds_depth must be 8, 12, 16 or 24 (0 = keep)");
     if (t->pt_chunk_paths < 0 || t->pt_chunk_paths > (int64_t(1) << 28))
@@ -469,8 +469,8 @@ MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcr
             r = d_rays.as<float>();
         }
         if ((rc = d_hits.ensure(n * sizeof(mcrt_hit)))) return rc;
-        static const int kWaves[] = {8, 12, 16, 20, 24, 32};
-        static const int kDepth[] = {8, 12, 16, 24};
+        static const int kWaves[] = {32, 64, 128, 256, 512};
+        static const int kDepth[] = {8, 12, 16};
         int bw = s->trace_waves_per_cu, bd = s->trace_lds_depth;
         for (int w : kWaves)
             for (int d : kDepth)

@@ -124,10 +124,12 @@ struct mcrt_scene
     std::mutex q_mu;
     mcrt::DevBuf q_in, q_out;
     // launch tuning (mcrt_scene_set_tuning / mcrt_autotune)
-    // defaults: the fastest closest-hit setting on MI355X for the 1.3M-triangle
-    // displaced-icosphere scene (24 x 6 KB LDS and 24 x 68 VGPRs fit one CU)
-    int32_t trace_waves_per_cu = 24;
-    int32_t trace_lds_depth = 12;
+    // defaults measured on MI355X (1.3M-triangle displaced icosphere, 16M
+    // incoherent rays): 8-entry LDS stacks let 32 waves (62 VGPRs each) be
+    // resident per CU, and 256 workgroups per CU -- 8 times what fits --
+    // leave the hardware scheduler to balance the ragged tail of the lanes
+    int32_t trace_waves_per_cu = 256;
+    int32_t trace_lds_depth = 8;
     int64_t pt_chunk_paths = int64_t(1) << 24;
 };
 

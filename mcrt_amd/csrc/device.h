@@ -33,7 +33,7 @@ struct DevCamera
 
 struct DevScene
 {
-    const WNode* nodes = nullptr;
+    const QNode* nodes = nullptr;
     const WRef* refs = nullptr;
     const TriShade* shade = nullptr;   // per triangle
     const float* verts = nullptr;      // per triangle a, b, c (9 floats)
@@ -54,7 +54,7 @@ struct DevScene
 // The part of a scene a traversal reads (batched launches look it up per ray).
 struct SceneRoot
 {
-    const WNode* nodes = nullptr;
+    const QNode* nodes = nullptr;
     const WRef* refs = nullptr;
 };
 

@@ -4,7 +4,9 @@
 //
 //   WNode (128 B): four child boxes as x/y/z lo/hi float4 rows, four child
 //                  links, and the octant visit-order table that reproduces
-//                  the binary tree's near-child-first order.
+//                  the binary tree's near-child-first order (host side).
+//   QNode ( 64 B): the WNode with outward-quantised child boxes; what the
+//                  kernels traverse.
 //   WRef  ( 64 B): one leaf reference = clipped box + the triangle as
 //                  v0, e1 = v1 - v0, e2 = v2 - v0 (the reference computes
 //                  the same two differences in MT, so storing them is
@@ -34,6 +36,33 @@ struct alignas(128) WNode
     uint32_t pad;
 };
 static_assert(sizeof(WNode) == 128, "WNode must be 128 bytes");
+
+// The node the kernels read: a WNode with its child boxes quantised to 8
+// bits per plane, rounded outwards, in 64 B (four 16 B rows):
+//   row 0: origin x, y, z (float) | scale exponents ex | ey << 8 | ez << 16
+//          | child count << 24
+//   row 1: child[4] (links as in WNode)
+//   row 2: lo_x, hi_x, lo_y, hi_y   (byte c of each word = child c)
+//   row 3: lo_z, hi_z, order_lo, order_hi
+// A plane decodes as origin + (float)q * 2^(e - 127) (q * 2^k is exact, one
+// rounding in the addition); the host chooses q so that every decoded box
+// contains the exact child box.  A contained box's slab interval is
+// contained after rounding, so a quantised test passes whenever the exact
+// one does; the extra subtrees it admits hold only references whose exact
+// clipped boxes fail, so results are those of the exact boxes.
+struct alignas(64) QNode
+{
+    float origin[3];
+    uint32_t exps;
+    uint32_t child[4];
+    uint32_t q_lo_x, q_hi_x, q_lo_y, q_hi_y;
+    uint32_t q_lo_z, q_hi_z;
+    uint32_t order_lo, order_hi;
+};
+static_assert(sizeof(QNode) == 64, "QNode must be 64 bytes");
+
+// Decoded plane of a QNode (the kernels evaluate the same expression).
+inline float qnode_plane(float origin, uint32_t q, float scale) { return origin + (float)q * scale; }
 
 struct alignas(64) WRef
 {

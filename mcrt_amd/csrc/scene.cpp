@@ -202,7 +202,7 @@ int finalize_scene(mcrt_scene* s, int threads)
     }
 
     int rc;
-    if ((rc = s->d_nodes.upload(wb.nodes.data(), wb.nodes.size() * sizeof(WNode)))) return rc;
+    if ((rc = s->d_nodes.upload(wb.qnodes.data(), wb.qnodes.size() * sizeof(QNode)))) return rc;
     if ((rc = s->d_refs.upload(wb.refs.data(), wb.refs.size() * sizeof(WRef)))) return rc;
     if ((rc = s->d_shade.upload(shade.data(), shade.size() * sizeof(TriShade)))) return rc;
     if ((rc = s->d_verts.upload(verts.data(), verts.size() * sizeof(float)))) return rc;
@@ -211,7 +211,7 @@ int finalize_scene(mcrt_scene* s, int threads)
     if ((rc = upload_materials(s))) return rc;
 
     DevScene& d = s->dev;
-    d.nodes = wb.nodes.empty() ? nullptr : s->d_nodes.as<WNode>();
+    d.nodes = wb.qnodes.empty() ? nullptr : s->d_nodes.as<QNode>();
     d.refs = s->d_refs.as<WRef>();
     d.shade = s->d_shade.as<TriShade>();
     d.verts = s->d_verts.as<float>();

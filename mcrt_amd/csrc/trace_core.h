@@ -91,6 +91,22 @@ MCRT_HD bool slab(const RayPrep& r, float lx, float hx, float ly, float hy, floa
     return true;
 }
 
+// A QNode plane (layout.h: qnode_plane): origin + q * 2^k, q one byte.
+MCRT_HD float qplane(float origin, uint32_t word, int shift, float scale)
+{
+    return origin + (float)((word >> shift) & 0xffu) * scale;
+}
+
+// Slab test of child `shift / 8` of a QNode against its decoded
+// (outward-rounded) box.
+MCRT_HD bool qslab(const RayPrep& r, uint32_t child, const uint4& qa, const uint4& qb, float ox, float oy, float oz,
+                   float sx, float sy, float sz, int shift, float R1, float& lo)
+{
+    if (child == MCRT_CHILD_EMPTY) return false;
+    return slab(r, qplane(ox, qa.x, shift, sx), qplane(ox, qa.y, shift, sx), qplane(oy, qa.z, shift, sy),
+                qplane(oy, qa.w, shift, sy), qplane(oz, qb.x, shift, sz), qplane(oz, qb.y, shift, sz), R1, lo);
+}
+
 // rayTriangleIntersect (triangle.cpp:13-51), e1/e2 precomputed.
 MCRT_HD bool mt(const RayPrep& r, float v0x, float v0y, float v0z, float e1x, float e1y,
                                    float e1z, float e2x, float e2y, float e2z, float& t, float& u, float& v)
@@ -185,17 +201,34 @@ struct Tracer
     {
         if (!(cur & MCRT_CHILD_LEAF))
         {
-            const float4* p = reinterpret_cast<const float4*>(S.nodes + cur);
-            const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
-            const uint4 ch = reinterpret_cast<const uint4*>(p)[6];
-            const uint2 meta = reinterpret_cast<const uint2*>(p)[14];  // order_lo, order_hi
+            // QNode (layout.h): four 16 B rows, child planes origin + q * 2^k
+            const uint4* p = reinterpret_cast<const uint4*>(S.nodes + cur);
+            const uint4 hd = p[0], ch = p[1], qa = p[2], qb = p[3];
+            const float ox = float_of(hd.x), oy = float_of(hd.y), oz = float_of(hd.z);
+            const float sx = float_of((hd.w & 0xffu) << 23);
+            const float sy = float_of(((hd.w >> 8) & 0xffu) << 23);
+            const float sz = float_of(((hd.w >> 16) & 0xffu) << 23);
             if (COUNT) cnt.nodes++;
             float l0, l1, l2, l3;
-            bool p0 = ch.x != MCRT_CHILD_EMPTY && slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, R1, l0);
-            bool p1 = ch.y != MCRT_CHILD_EMPTY && slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, R1, l1);
-            bool p2 = ch.z != MCRT_CHILD_EMPTY && slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, R1, l2);
-            bool p3 = ch.w != MCRT_CHILD_EMPTY && slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, R1, l3);
-            const uint32_t ord = ((r.oct & 4u) ? meta.y : meta.x) >> ((r.oct & 3u) * 8u);
+            bool p0, p1, p2, p3;
+            if (!r.exact)
+            {
+                p0 = qslab(r, ch.x, qa, qb, ox, oy, oz, sx, sy, sz, 0, R1, l0);
+                p1 = qslab(r, ch.y, qa, qb, ox, oy, oz, sx, sy, sz, 8, R1, l1);
+                p2 = qslab(r, ch.z, qa, qb, ox, oy, oz, sx, sy, sz, 16, R1, l2);
+                p3 = qslab(r, ch.w, qa, qb, ox, oy, oz, sx, sy, sz, 24, R1, l3);
+            }
+            else
+            {
+                // an infinite inv_dir component: descend every child and let
+                // the exact reference tests decide (results are unchanged)
+                p0 = ch.x != MCRT_CHILD_EMPTY;
+                p1 = ch.y != MCRT_CHILD_EMPTY;
+                p2 = ch.z != MCRT_CHILD_EMPTY;
+                p3 = ch.w != MCRT_CHILD_EMPTY;
+                l0 = l1 = l2 = l3 = -__builtin_inff();
+            }
+            const uint32_t ord = ((r.oct & 4u) ? qb.w : qb.z) >> ((r.oct & 3u) * 8u);
             uint32_t next = MCRT_CHILD_EMPTY;
             float next_lo = 0.0f;
 #pragma unroll

@@ -12,7 +12,9 @@
 #include "wide.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <limits>
 
 namespace mcrt
 {
@@ -190,11 +192,112 @@ void collapse_wide(const BTree& bt, const std::vector<Tri>& tris, WideBVH& out)
         }
         out.stack_bound = 1;
         out.depth = 1;
+        quantize_nodes(out);
         return;
     }
     std::vector<TL> tl{TL{root.axis, -1, -2}};
     b.make({root.left, root.right}, tl, need, depth);
     out.stack_bound = need;
     out.depth = depth;
+    quantize_nodes(out);
+}
+
+namespace
+{
+// One axis of a node: origin = lowest child plane; scale 2^k with the
+// smallest k in [-126, 127] at which every child's planes round outwards
+// within 8 bits (checked with the kernels' decode expression).
+bool quantize_axis(const float* lo, const float* hi, const bool* valid, float& origin, uint32_t& ebyte,
+                   uint32_t& qlo, uint32_t& qhi)
+{
+    bool any = false;
+    float o = 0.0f, top = 0.0f;
+    for (int c = 0; c < 4; c++)
+        if (valid[c])
+        {
+            o = any ? std::min(o, lo[c]) : lo[c];
+            top = any ? std::max(top, hi[c]) : hi[c];
+            any = true;
+        }
+    origin = o;
+    qlo = qhi = 0;
+    if (!any)
+    {
+        ebyte = 127;
+        return true;
+    }
+    const double ext = (double)top - (double)o;
+    int k = ext > 0.0 ? (int)std::ceil(std::log2(ext / 250.0)) : -126;
+    k = std::max(k, -126);
+    for (; k <= 127; k++)
+    {
+        const float sc = std::ldexp(1.0f, k);
+        uint32_t ql = 0, qh = 0;
+        bool ok = true;
+        for (int c = 0; c < 4 && ok; c++)
+        {
+            if (!valid[c]) continue;
+            double a = std::floor(((double)lo[c] - (double)o) / sc);
+            uint32_t l = (uint32_t)std::min(255.0, std::max(0.0, a));
+            while (l > 0 && qnode_plane(o, l, sc) > lo[c]) l--;
+            double b = std::ceil(((double)hi[c] - (double)o) / sc);
+            if (!(b <= 255.0))
+            {
+                ok = false;
+                break;
+            }
+            uint32_t h = (uint32_t)std::max(0.0, b);
+            while (h < 255 && qnode_plane(o, h, sc) < hi[c]) h++;
+            if (qnode_plane(o, l, sc) > lo[c] || qnode_plane(o, h, sc) < hi[c])
+            {
+                ok = false;
+                break;
+            }
+            ql |= l << (8 * c);
+            qh |= h << (8 * c);
+        }
+        if (ok)
+        {
+            ebyte = (uint32_t)(k + 127);
+            qlo = ql;
+            qhi = qh;
+            return true;
+        }
+    }
+    return false;
+}
+}  // namespace
+
+void quantize_nodes(WideBVH& out)
+{
+    out.qnodes.resize(out.nodes.size());
+    for (size_t i = 0; i < out.nodes.size(); i++)
+    {
+        const WNode& w = out.nodes[i];
+        QNode& q = out.qnodes[i];
+        std::memset(&q, 0, sizeof(q));
+        bool valid[4];
+        for (int c = 0; c < 4; c++)
+        {
+            valid[c] = w.child[c] != MCRT_CHILD_EMPTY;
+            q.child[c] = w.child[c];
+        }
+        uint32_t e[3];
+        const bool ok = quantize_axis(w.lo_x, w.hi_x, valid, q.origin[0], e[0], q.q_lo_x, q.q_hi_x) &&
+                        quantize_axis(w.lo_y, w.hi_y, valid, q.origin[1], e[1], q.q_lo_y, q.q_hi_y) &&
+                        quantize_axis(w.lo_z, w.hi_z, valid, q.origin[2], e[2], q.q_lo_z, q.q_hi_z);
+        if (!ok)
+        {
+            // only non-finite boxes lack a conservative grid (any finite box
+            // set fits at scale 2^127): planes [-FLT_MAX, +inf] on every axis
+            for (int a = 0; a < 3; a++) q.origin[a] = -std::numeric_limits<float>::max();
+            e[0] = e[1] = e[2] = 254;
+            q.q_lo_x = q.q_lo_y = q.q_lo_z = 0;
+            q.q_hi_x = q.q_hi_y = q.q_hi_z = 0xffffffffu;
+        }
+        q.exps = e[0] | (e[1] << 8) | (e[2] << 16) | (w.nchild << 24);
+        q.order_lo = w.order_lo;
+        q.order_hi = w.order_hi;
+    }
 }
 }  // namespace mcrt

@@ -10,10 +10,14 @@ namespace mcrt
 struct WideBVH
 {
     std::vector<WNode> nodes;  // nodes[0] = root (absent if no triangles)
+    std::vector<QNode> qnodes; // nodes quantised for the kernels (same indices)
     std::vector<WRef> refs;
     int32_t stack_bound = 0;   // stack entries a traversal can ever hold
     int32_t depth = 0;         // wide levels
 };
 
 void collapse_wide(const BTree& bt, const std::vector<Tri>& tris, WideBVH& out);
+
+// Fills out.qnodes from out.nodes (collapse_wide calls it).
+void quantize_nodes(WideBVH& out);
 }  // namespace mcrt

@@ -82,6 +82,7 @@ def hostemu():
         "emu_build": (P, [P, LL, I]),
         "emu_digest": (U64, [P, P]),
         "emu_free": (None, [P]),
+        "emu_check_qnodes": (None, [P, P]),
         "emu_trace": (None, [P, P, LL, P, P, I, I, P, I]),
     })
 

@@ -63,6 +63,49 @@ extern "C" unsigned long long emu_digest(void* h, long long* stats)
 
 extern "C" void emu_free(void* h) { delete (Emu*)h; }
 
+// Quantised nodes against the exact ones: out[0] = planes checked, out[1] =
+// decoded planes that do not contain the exact child box, out[2] = child
+// links or order tables that differ, out[3] = sum over checked planes of
+// (decoded - exact) outward growth relative to the node extent, x 1e6.
+extern "C" void emu_check_qnodes(void* h, long long* out)
+{
+    Emu* e = (Emu*)h;
+    long long checked = 0, bad = 0, links = 0;
+    double grow = 0.0;
+    for (size_t i = 0; i < e->wb.nodes.size(); i++)
+    {
+        const mcrt::WNode& w = e->wb.nodes[i];
+        const mcrt::QNode& q = e->wb.qnodes[i];
+        if (q.order_lo != w.order_lo || q.order_hi != w.order_hi) links++;
+        const float* lo[3] = {w.lo_x, w.lo_y, w.lo_z};
+        const float* hi[3] = {w.hi_x, w.hi_y, w.hi_z};
+        const uint32_t qlo[3] = {q.q_lo_x, q.q_lo_y, q.q_lo_z};
+        const uint32_t qhi[3] = {q.q_hi_x, q.q_hi_y, q.q_hi_z};
+        for (int c = 0; c < 4; c++)
+        {
+            if (q.child[c] != w.child[c]) links++;
+            if (w.child[c] == MCRT_CHILD_EMPTY) continue;
+            for (int a = 0; a < 3; a++)
+            {
+                const float sc = mcrt::float_of(((q.exps >> (8 * a)) & 0xffu) << 23);
+                const float dl = mcrt::qnode_plane(q.origin[a], (qlo[a] >> (8 * c)) & 0xffu, sc);
+                const float dh = mcrt::qnode_plane(q.origin[a], (qhi[a] >> (8 * c)) & 0xffu, sc);
+                checked += 2;
+                if (!(dl <= lo[a][c])) bad++;
+                if (!(dh >= hi[a][c])) bad++;
+                float ext = 0.0f;
+                for (int k = 0; k < 4; k++)
+                    if (w.child[k] != MCRT_CHILD_EMPTY) ext = std::max(ext, hi[a][k] - q.origin[a]);
+                if (ext > 0.0f) grow += ((double)(lo[a][c] - dl) + (double)(dh - hi[a][c])) / ext;
+            }
+        }
+    }
+    out[0] = checked;
+    out[1] = bad;
+    out[2] = links;
+    out[3] = (long long)(grow * 1e6);
+}
+
 // rays n x 8; hits n x 4 int32 {prim, t, u, v}; occ n bytes (any-hit);
 // counters 3 totals (node fetches, ref fetches, MT tests).
 extern "C" void emu_trace(void* h, const float* rays, long long n, int* hits, unsigned char* occ, int any,
@@ -70,7 +113,7 @@ extern "C" void emu_trace(void* h, const float* rays, long long n, int* hits, un
 {
     Emu* e = (Emu*)h;
     DevScene S;
-    S.nodes = e->wb.nodes.empty() ? nullptr : e->wb.nodes.data();
+    S.nodes = e->wb.qnodes.empty() ? nullptr : e->wb.qnodes.data();
     S.refs = e->wb.refs.data();
     S.stack_bound = e->wb.stack_bound;
     // std::thread, not OpenMP: this TU is compiled by hipcc, whose OpenMP

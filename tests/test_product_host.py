@@ -103,17 +103,51 @@ def test_degenerate_and_coplanar_inputs(emu):
     assert np.array_equal(hits[:, 1:].view(np.uint32), tuv.view(np.uint32))
 
 
-def test_visit_counts_match_oracle_refs(emu):
-    """The instrumented traversal visits at most as many references as the
-    binary traversal tests leaf slots (the 4-wide collapse only removes the
-    redundant inner-box and leaf-box tests)."""
-    tris = _libs.debug_mesh()
-    rays = _libs.random_rays(tris, 1 << 12, seed=4)
+@pytest.mark.parametrize("which", ["debug_mesh", "cornell_ico4"])
+def test_visit_counts_vs_oracle(emu, which):
+    """The 4-wide traversal over outward-quantised node boxes runs exactly
+    the reference's Moller-Trumbore tests: collapsing drops only implied
+    inner-box tests, and a quantised box that passes where the exact one
+    fails admits only leaf references whose exact clipped boxes then fail.
+    So MT tests are equal, reference box tests can only grow (and only
+    slightly: 8-bit planes), and node fetches stay below the binary
+    traversal's box tests."""
+    if which == "debug_mesh":
+        tris = _libs.debug_mesh()
+        rays = _libs.random_rays(tris, 1 << 12, seed=4)
+    else:
+        from mcrt_amd import scenes
+        tris = scenes.cornell_icosphere(4).tris
+        rays = scenes.incoherent_rays(1 << 13, lo=np.array([1.0, 1.0, 1.0]), hi=np.array([54.0, 53.0, 54.0]),
+                                      seed=8)
     _, cnt = Emu(tris).trace(rays)
     _, _, oc = _libs.OracleBVH(tris).intersect(rays, counters=True)
-    assert cnt[1] == oc[1]  # same leaf references box-tested
-    assert cnt[2] == oc[2]  # same Moller-Trumbore tests
-    assert cnt[0] < oc[0]   # fewer node fetches than binary box tests
+    assert cnt[2] == oc[2]                  # same Moller-Trumbore tests
+    assert oc[1] <= cnt[1] <= 1.1 * oc[1]   # reference box tests: only the quantisation's extras
+    assert cnt[0] < oc[0]                   # fewer node fetches than binary box tests
+
+
+@pytest.mark.parametrize("which", ["debug_mesh", "soup_slice", "cornell_displaced_f40"])
+def test_quantised_nodes_contain_exact_boxes(emu, which):
+    """Every decoded QNode child plane (origin + q * 2^k, the kernels'
+    expression) lies outside or on the exact WNode child box, child links
+    and octant order tables are carried over unchanged, and the outward
+    growth stays small (8-bit planes: about 1/250 of the node extent at
+    most, well under that on average)."""
+    if which == "debug_mesh":
+        tris = _libs.debug_mesh()
+    elif which == "soup_slice":
+        tris = _libs.soup_slice()
+    else:
+        from mcrt_amd import scenes
+        tris = scenes.cornell_displaced(geodesic_freq=40).tris
+    e = Emu(tris)
+    out = np.zeros(4, np.int64)
+    e.L.emu_check_qnodes(e.h, ptr(out))
+    checked, bad, links, grow = (int(x) for x in out)
+    assert checked > 0
+    assert bad == 0 and links == 0
+    assert grow / 1e6 / checked < 2.0 / 250.0  # mean growth per plane pair, relative to the node extent
 
 
 def test_c3_soup_full_size_builder_matches_oracle(emu):
