@@ -168,14 +168,25 @@ def c3_leg(ctx, steps):
     import torch
     import mcrt_amd
     from mcrt_amd import scenes
+    import tempfile
     tris = scenes.c3_soup()
-    s = mcrt_amd.Scene(ctx)
-    s.add(tris, s.add_material(mcrt_amd.DIFF, (0.5, 0.5, 0.5)))
-    st = s.finalize()
-    log(f"C3 soup SBVH: {st['build_seconds']:.2f} s build, {st['refs']} refs")
+    with tempfile.TemporaryDirectory() as tmp:
+        cache = os.path.join(tmp, "soup.mcrtbvh")
+        s0 = mcrt_amd.Scene(ctx)
+        s0.add(tris, s0.add_material(mcrt_amd.DIFF, (0.5, 0.5, 0.5)))
+        st = s0.finalize(bvh_cache=cache)  # builds and writes the flattened BVH
+        s0.close()
+        cache_mb = os.path.getsize(cache) / 1e6
+        s = mcrt_amd.Scene(ctx)
+        s.add(tris, s.add_material(mcrt_amd.DIFF, (0.5, 0.5, 0.5)))
+        st2 = s.finalize(bvh_cache=cache)  # loads it
+    assert st2["from_cache"] == 1 and st2["tree_digest"] == st["tree_digest"]
+    log(f"C3 soup SBVH: {st['build_seconds']:.2f} s build, {st['refs']} refs; cache load {st2['build_seconds']:.2f} s")
     out = {"config": {"triangles": int(tris.shape[0]), "rays": scenes.C3_RAYS, "refs": st["refs"],
                       "wide_nodes": st["wide_nodes"], "sbvh_build_s": round(st["build_seconds"], 2),
-                      "any_hit_tfar": scenes.C3_ANY_TFAR}}
+                      "any_hit_tfar": scenes.C3_ANY_TFAR},
+           "bvh_cache": {"build_and_write_s": round(st["build_seconds"], 2),
+                         "load_s": round(st2["build_seconds"], 2), "file_mb": round(cache_mb, 1)}}
     n = scenes.C3_RAYS
     hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
     occ = torch.empty(n, dtype=torch.uint8, device="cuda")

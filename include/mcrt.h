@@ -70,7 +70,12 @@ typedef struct mcrt_hit {
 } mcrt_hit;
 
 typedef struct mcrt_build_opts {
-    int32_t threads; /* host threads for the SBVH build; 0 = all cores      */
+    int32_t threads;       /* host threads for the SBVH build; 0 = all cores  */
+    int32_t reserved;
+    const char* bvh_cache; /* optional file: when it holds the flattened BVH of
+                              these exact triangles (same bits, same order) it
+                              is loaded instead of building; otherwise the BVH
+                              is built and the file (re)written.  NULL = build */
 } mcrt_build_opts;
 
 typedef struct mcrt_build_stats {
@@ -85,6 +90,7 @@ typedef struct mcrt_build_stats {
     double build_seconds;   /* host SBVH build                               */
     double flatten_seconds; /* collapse + upload                             */
     int64_t device_bytes;   /* scene bytes resident in HBM                   */
+    int64_t from_cache;     /* 1: the BVH was loaded from opts->bvh_cache    */
 } mcrt_build_stats;
 
 typedef struct mcrt_render_params {

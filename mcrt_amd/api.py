@@ -31,14 +31,14 @@ class BuildStats(ctypes.Structure):
                 ("refs", ctypes.c_int64), ("wide_nodes", ctypes.c_int64), ("max_depth", ctypes.c_int64),
                 ("stack_bound", ctypes.c_int64), ("tree_digest", ctypes.c_uint64),
                 ("build_seconds", ctypes.c_double), ("flatten_seconds", ctypes.c_double),
-                ("device_bytes", ctypes.c_int64)]
+                ("device_bytes", ctypes.c_int64), ("from_cache", ctypes.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class BuildOpts(ctypes.Structure):
-    _fields_ = [("threads", ctypes.c_int32)]
+    _fields_ = [("threads", ctypes.c_int32), ("reserved", ctypes.c_int32), ("bvh_cache", ctypes.c_char_p)]
 
 
 class RenderParams(ctypes.Structure):
@@ -217,9 +217,11 @@ class Scene:
                                            float(fov_y)))
         self.camera = (tuple(o.tolist()), tuple(t.tolist()), int(width), int(height), float(fov_y))
 
-    def finalize(self, threads=0):
+    def finalize(self, threads=0, bvh_cache=None):
+        """Scene::finalize; bvh_cache = optional file of the flattened BVH
+        (loaded when it matches these triangles, else written)."""
         st = BuildStats()
-        opts = BuildOpts(threads)
+        opts = BuildOpts(threads, 0, None if bvh_cache is None else os.fsencode(bvh_cache))
         _check(lib().mcrt_scene_finalize(self.h, ctypes.byref(opts), ctypes.byref(st)))
         self.stats = st.as_dict()
         return self.stats

@@ -293,7 +293,7 @@ MCRT_API int mcrt_scene_finalize(mcrt_scene* s, const mcrt_build_opts* opts, mcr
     if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_finalize: NULL scene");
     if (s->finalized) return fail(MCRT_ERR_STATE, "mcrt_scene_finalize: scene already finalized");
     HIPCHK(hipSetDevice(s->ctx->device));
-    int rc = finalize_scene(s, opts ? opts->threads : 0);
+    int rc = finalize_scene(s, opts ? opts->threads : 0, opts ? opts->bvh_cache : nullptr);
     if (rc) return rc;
     s->finalized = true;
     if (stats) *stats = s->stats;

@@ -136,7 +136,9 @@ struct mcrt_scene
 namespace mcrt
 {
 DevCamera make_camera(const mcrt_scene* s);
-int finalize_scene(mcrt_scene* s, int threads);
+// Builds (or, with `cache`, loads when it matches) the BVH, collapses and
+// uploads the scene.
+int finalize_scene(mcrt_scene* s, int threads, const char* cache);
 int upload_materials(mcrt_scene* s);
 int run_query(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, uint8_t* occ, int mode, int flags,
               hipStream_t stream, uint64_t* counters);

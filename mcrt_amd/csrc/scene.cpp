@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 
@@ -109,25 +111,154 @@ int upload_materials(mcrt_scene* s)
     return MCRT_OK;
 }
 
-int finalize_scene(mcrt_scene* s, int threads)
+// ------------------------------------------------------------ BVH cache --
+// File: CacheHeader, then the QNode array, then the WRef array.  A file is
+// used only when magic, version, record sizes, triangle count, the hash of
+// the triangles' bits, the file size and the hash of the payload all match.
+namespace
+{
+struct CacheHeader
+{
+    char magic[8];
+    uint32_t version, qnode_size, wref_size, pad;
+    uint64_t ntri, tri_hash, nq, nr;
+    int32_t stack_bound, depth;
+    int64_t binary_nodes, leaves, refs, max_depth;
+    uint64_t tree_digest, payload_hash;
+};
+const char kCacheMagic[8] = {'M', 'C', 'R', 'T', 'B', 'V', 'H', '\0'};
+const uint32_t kCacheVersion = 1;
+
+// FNV-1a over 64-bit words (bytes beyond a multiple of 8 folded in last).
+uint64_t hash_bytes(const void* p, size_t n, uint64_t h = 0xcbf29ce484222325ull)
+{
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8)
+    {
+        uint64_t w;
+        std::memcpy(&w, b + i, 8);
+        h = (h ^ w) * 0x100000001b3ull;
+    }
+    for (; i < n; i++) h = (h ^ b[i]) * 0x100000001b3ull;
+    return h;
+}
+
+uint64_t triangles_hash(const std::vector<Tri>& tris)
+{
+    return hash_bytes(tris.data(), tris.size() * sizeof(Tri), 0xcbf29ce484222325ull ^ (uint64_t)tris.size());
+}
+
+bool load_bvh_cache(const char* path, uint64_t ntri, uint64_t tri_hash, WideBVH& wb, mcrt_build_stats& st)
+{
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    CacheHeader h;
+    bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, kCacheMagic, 8) == 0 &&
+              h.version == kCacheVersion && h.qnode_size == sizeof(QNode) && h.wref_size == sizeof(WRef) &&
+              h.ntri == ntri && h.tri_hash == tri_hash;
+    if (ok)
+    {
+        std::fseek(f, 0, SEEK_END);
+        const long size = std::ftell(f);
+        ok = size == (long)(sizeof(h) + h.nq * sizeof(QNode) + h.nr * sizeof(WRef));
+        std::fseek(f, (long)sizeof(h), SEEK_SET);
+    }
+    if (ok)
+    {
+        wb.qnodes.resize(h.nq);
+        wb.refs.resize(h.nr);
+        ok = std::fread(wb.qnodes.data(), sizeof(QNode), h.nq, f) == h.nq &&
+             std::fread(wb.refs.data(), sizeof(WRef), h.nr, f) == h.nr;
+        ok = ok && hash_bytes(wb.refs.data(), h.nr * sizeof(WRef),
+                              hash_bytes(wb.qnodes.data(), h.nq * sizeof(QNode))) == h.payload_hash;
+    }
+    std::fclose(f);
+    if (!ok)
+    {
+        wb.qnodes.clear();
+        wb.refs.clear();
+        return false;
+    }
+    wb.stack_bound = h.stack_bound;
+    wb.depth = h.depth;
+    st.binary_nodes = h.binary_nodes;
+    st.leaves = h.leaves;
+    st.refs = h.refs;
+    st.max_depth = h.max_depth;
+    st.tree_digest = h.tree_digest;
+    return true;
+}
+
+// Writes to path + ".tmp" and renames, so readers never see a partial file.
+int store_bvh_cache(const char* path, uint64_t ntri, uint64_t tri_hash, const WideBVH& wb,
+                    const mcrt_build_stats& st)
+{
+    CacheHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, kCacheMagic, 8);
+    h.version = kCacheVersion;
+    h.qnode_size = sizeof(QNode);
+    h.wref_size = sizeof(WRef);
+    h.ntri = ntri;
+    h.tri_hash = tri_hash;
+    h.nq = wb.qnodes.size();
+    h.nr = wb.refs.size();
+    h.stack_bound = wb.stack_bound;
+    h.depth = wb.depth;
+    h.binary_nodes = st.binary_nodes;
+    h.leaves = st.leaves;
+    h.refs = st.refs;
+    h.max_depth = st.max_depth;
+    h.tree_digest = st.tree_digest;
+    h.payload_hash = hash_bytes(wb.refs.data(), h.nr * sizeof(WRef), hash_bytes(wb.qnodes.data(), h.nq * sizeof(QNode)));
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return fail(MCRT_ERR_IO, "cannot write BVH cache %s", tmp.c_str());
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 &&
+              std::fwrite(wb.qnodes.data(), sizeof(QNode), h.nq, f) == h.nq &&
+              std::fwrite(wb.refs.data(), sizeof(WRef), h.nr, f) == h.nr;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0)
+    {
+        std::remove(tmp.c_str());
+        return fail(MCRT_ERR_IO, "cannot write BVH cache %s", path);
+    }
+    return MCRT_OK;
+}
+}  // namespace
+
+int finalize_scene(mcrt_scene* s, int threads, const char* cache)
 {
     using clk = std::chrono::steady_clock;
     const size_t n = s->tris.size();
     mcrt_build_stats st{};
     st.triangles = (int64_t)n;
     auto t0 = clk::now();
-    BTree bt;
-    build_sbvh(s->tris, threads, bt);
-    auto t1 = clk::now();
-    st.binary_nodes = (int64_t)bt.nodes.size();
-    for (const BNode& b : bt.nodes) st.leaves += b.leaf ? 1 : 0;
-    st.refs = (int64_t)bt.refs.size();
-    st.max_depth = bt.max_depth;
-    st.tree_digest = tree_digest(bt);
     WideBVH wb;
-    collapse_wide(bt, s->tris, wb);
-    { BTree().nodes.swap(bt.nodes); std::vector<SRef>().swap(bt.refs); }
-    st.wide_nodes = (int64_t)wb.nodes.size();
+    const uint64_t th = cache ? triangles_hash(s->tris) : 0;
+    if (cache && load_bvh_cache(cache, n, th, wb, st))
+        st.from_cache = 1;
+    else
+    {
+        BTree bt;
+        build_sbvh(s->tris, threads, bt);
+        st.binary_nodes = (int64_t)bt.nodes.size();
+        for (const BNode& b : bt.nodes) st.leaves += b.leaf ? 1 : 0;
+        st.refs = (int64_t)bt.refs.size();
+        st.max_depth = bt.max_depth;
+        st.tree_digest = tree_digest(bt);
+        collapse_wide(bt, s->tris, wb);
+        { BTree().nodes.swap(bt.nodes); std::vector<SRef>().swap(bt.refs); }
+        std::vector<WNode>().swap(wb.nodes);
+        if (cache)
+        {
+            const int rc = store_bvh_cache(cache, n, th, wb, st);
+            if (rc) return rc;
+        }
+    }
+    auto t1 = clk::now();
+    st.wide_nodes = (int64_t)wb.qnodes.size();
     st.stack_bound = wb.stack_bound;
 
     // Per-triangle shading records and the light list (Scene::finalize,

@@ -171,7 +171,7 @@ class SceneData:
     def ntri(self):
         return self.tris.shape[0]
 
-    def to_scene(self, ctx, width=None, height=None, camera=CAMERA, finalize=True, threads=0):
+    def to_scene(self, ctx, width=None, height=None, camera=CAMERA, finalize=True, threads=0, bvh_cache=None):
         """Builds an mcrt Scene (materials, triangles, camera) and finalizes it."""
         s = api.Scene(ctx)
         s.add_materials(self.mat_type, self.mat_rgb, self.mat_gloss)
@@ -179,7 +179,7 @@ class SceneData:
         if width:
             s.set_camera(camera["origin"], camera["target"], width, height, camera["fov_y"])
         if finalize:
-            s.finalize(threads)
+            s.finalize(threads, bvh_cache=bvh_cache)
         return s
 
 
