@@ -267,6 +267,42 @@ def inverse_rendering(freq=71, hidden_seed=5):
     return target, init_rgb
 
 
+def sky_envmap(width=128, height=64, seed=5, sun=(0.35, 0.3), sun_radiance=60.0):
+    """Synthetic latitude-longitude environment (rows top-down: row v <->
+    polar angle pi v from +y, column u <-> azimuth 2 pi u): a blue-to-white
+    sky gradient above the horizon, a darker ground, a small bright sun disc
+    at (u, v) = `sun` and low-amplitude value noise (seed).  (H, W, 3) float32."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    U, V = np.meshgrid(u, v)
+    up = np.clip(1.0 - 2.0 * V, -1.0, 1.0)  # cos(polar angle)
+    sky = np.stack([0.35 + 0.5 * (1 - up), 0.55 + 0.35 * (1 - up), 0.95 + 0.05 * (1 - up)], -1) * (up > 0)[..., None]
+    ground = np.stack([0.25, 0.2, 0.15], 0)[None, None, :] * (up <= 0)[..., None]
+    rng = np.random.default_rng(seed)
+    noise = 1.0 + 0.1 * rng.uniform(-1, 1, (height, width, 1))
+    env = (sky + ground) * noise
+    du = np.minimum(np.abs(U - sun[0]), 1 - np.abs(U - sun[0])) * 2.0
+    dist = np.hypot(du, V - sun[1])
+    env = env + (dist < 0.04)[..., None] * np.array([sun_radiance, 0.9 * sun_radiance, 0.8 * sun_radiance])
+    return env.astype(np.float32)
+
+
+def write_envmap(path, rgb):
+    """EnvmapLoader's file format (envmap.cpp:98-118): int32 width, int32
+    height, then width*height RGB float32, rows top-down."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w, _ = rgb.shape
+    with open(path, "wb") as fh:
+        fh.write(np.array([w, h], np.int32).tobytes())
+        fh.write(rgb.tobytes())
+
+
+def read_envmap(path):
+    with open(path, "rb") as fh:
+        w, h = np.frombuffer(fh.read(8), np.int32)
+        return np.frombuffer(fh.read(), np.float32, count=int(w) * int(h) * 3).reshape(int(h), int(w), 3).copy()
+
+
 C4_SIDES = (128, 256, 384, 512)
 C4_SPP = (8, 16, 32, 64)
 C4_RENDER = dict(max_depth=6, rr_start=3, rr_prob=0.8, seed=1)

@@ -20,6 +20,7 @@
 #include "camera.h"
 #include "constants.h"
 #include "debug_mesh.h"
+#include "envmap.h"
 #include "film.h"
 #include "material.h"
 #include "mesh.h"
@@ -346,4 +347,35 @@ REF_API void ref_film_tonemap_gamma(float* film, int w, int h, float gamma)
             float* p = film + 3 * ((size_t)j * w + i);
             p[0] = c.x; p[1] = c.y; p[2] = c.z;
         }
+}
+
+// EnvmapLoader(path) of a `{int w, h}` + float RGB file.
+REF_API void* ref_envmap_create(const char* path) { return new EnvmapLoader(path); }
+REF_API void ref_envmap_free(void* h) { delete (EnvmapLoader*)h; }
+
+// getPdf and sampleRadiance of n directions: out n x 4 = {pdf, r, g, b}.
+REF_API void ref_envmap_query(void* h, const float* dirs, long long n, float* out4)
+{
+    const EnvmapLoader* e = (const EnvmapLoader*)h;
+    for (long long i = 0; i < n; i++)
+    {
+        float d[3] = {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+        out4[4 * i] = e->getPdf(d);
+        e->sampleRadiance(d, out4 + 4 * i + 1);
+    }
+}
+
+// sample(r1, r2) for n pairs (the sub-pixel jitter comes from the
+// reference's own randf()): out n x 7 = {dir, rad, pdf}.
+REF_API void ref_envmap_sample(void* h, const float* r2, long long n, float* out7)
+{
+    const EnvmapLoader* e = (const EnvmapLoader*)h;
+    for (long long i = 0; i < n; i++)
+    {
+        DirLight dl = e->sample(r2[2 * i], r2[2 * i + 1]);
+        float* o = out7 + 7 * i;
+        o[0] = dl.dir[0]; o[1] = dl.dir[1]; o[2] = dl.dir[2];
+        o[3] = dl.rad[0]; o[4] = dl.rad[1]; o[5] = dl.rad[2];
+        o[6] = dl.pdf;
+    }
 }

@@ -54,6 +54,10 @@ def oracle():
         "orc_render_adjoint": (None, [P, P, P, U64, P, P, P, P, I]),
         "orc_render_loss": (ctypes.c_double, [P, P, P, U64, P, I]),
         "orc_scene_set_rgb": (None, [P, I, P]),
+        "orc_scene_set_envmap": (None, [P, P, I, I]),
+        "orc_env_query": (None, [P, P, LL, I, P]),
+        "orc_env_sample": (None, [P, P, LL, P]),
+        "orc_env_tables": (ctypes.c_float, [P, P, P, P]),
     })
 
 
@@ -74,6 +78,10 @@ def reference():
         "ref_discrete": (None, [P, I, P, LL, P, P]),
         "ref_film_dump": (None, [ctypes.c_char_p, P, I, I, I]),
         "ref_film_tonemap_gamma": (None, [P, I, I, ctypes.c_float]),
+        "ref_envmap_create": (P, [ctypes.c_char_p]),
+        "ref_envmap_free": (None, [P]),
+        "ref_envmap_query": (None, [P, P, LL, P]),
+        "ref_envmap_sample": (None, [P, P, LL, P]),
     })
 
 
@@ -207,6 +215,35 @@ class OracleScene:
         ip, fp = self._params(w, h, spp, max_depth, rr_start, rr_prob, nee, 1, camera)
         weights = np.ascontiguousarray(weights, np.float32)
         return self.L.orc_render_loss(self.h, ptr(ip), ptr(fp), int(seed), ptr(weights), threads)
+
+    def set_envmap(self, rgb):
+        """Scene::setEnvmap with the (H, W, 3) radiance in memory."""
+        self.env = np.ascontiguousarray(rgb, np.float32)
+        h, w, _ = self.env.shape
+        self.L.orc_scene_set_envmap(self.h, ptr(self.env), w, h)
+
+    def env_query(self, dirs, precision=0):
+        """(pixel, pdf, rgb) of each direction (getPdf / sampleRadiance)."""
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        out = np.zeros((dirs.shape[0], 5), np.float32)
+        self.L.orc_env_query(self.h, ptr(dirs), dirs.shape[0], precision, ptr(out))
+        return out[:, 0].view(np.int32).copy(), out[:, 1].copy(), out[:, 2:].copy()
+
+    def env_sample(self, r4):
+        """EnvmapLoader::sample for rows (r1, r2, jitter_u, jitter_v): (dir, rad, pdf)."""
+        r4 = np.ascontiguousarray(r4, np.float32)
+        out = np.zeros((r4.shape[0], 7), np.float32)
+        self.L.orc_env_sample(self.h, ptr(r4), r4.shape[0], ptr(out))
+        return out[:, :3].copy(), out[:, 3:6].copy(), out[:, 6].copy()
+
+    def env_tables(self):
+        """(cdf_u (W,), cdf_v (W, H), lum (H, W), norm)."""
+        h, w, _ = self.env.shape
+        cu = np.zeros(w, np.float32)
+        cv = np.zeros((w, h), np.float32)
+        lum = np.zeros((h, w), np.float32)
+        norm = self.L.orc_env_tables(self.h, ptr(cu), ptr(cv), ptr(lum))
+        return cu, cv, lum, norm
 
     def set_rgb(self, mat, rgb):
         rgb = np.ascontiguousarray(rgb, np.float32)

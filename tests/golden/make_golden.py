@@ -169,6 +169,45 @@ def camera_fixture(R):
     np.savez_compressed(os.path.join(HERE, "camera.npz"), pxy=pxy, rays=cr)
 
 
+def envmap_inputs():
+    """Synthetic environment (64 x 32), directions at every pixel centre,
+    random directions and random sample pairs (seeded)."""
+    from mcrt_amd import scenes
+    env = scenes.sky_envmap(64, 32, seed=9)
+    h, w, _ = env.shape
+    v = (np.arange(h) + 0.5) / h
+    u = (np.arange(w) + 0.5) / w
+    U, V = np.meshgrid(u, v)
+    phi, theta = np.pi * V.ravel(), 2 * np.pi * U.ravel()
+    centres = np.stack([np.sin(phi) * np.sin(theta), np.cos(phi), -np.sin(phi) * np.cos(theta)], 1).astype(np.float32)
+    rng = np.random.default_rng(21)
+    d = rng.normal(size=(4096, 3))
+    rand = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    r2 = rng.uniform(0, 1, (4096, 2)).astype(np.float32)
+    return env, centres, rand, r2
+
+
+def envmap_fixture(R):
+    import tempfile
+    env, centres, rand, r2 = envmap_inputs()
+    from mcrt_amd import scenes
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "env.bin")
+        scenes.write_envmap(path, env)
+        h = R.ref_envmap_create(path.encode())
+        out = {}
+        for name, dirs in (("centres", centres), ("rand", rand)):
+            q = np.zeros((dirs.shape[0], 4), np.float32)
+            R.ref_envmap_query(h, ptr(dirs), dirs.shape[0], ptr(q))
+            out[name] = q
+        smp = np.zeros((r2.shape[0], 7), np.float32)
+        R.ref_envmap_sample(h, ptr(r2), r2.shape[0], ptr(smp))
+        R.ref_envmap_free(h)
+    np.savez_compressed(os.path.join(HERE, "envmap.npz"), env=env, centres=centres, rand=rand, r2=r2,
+                        q_centres=out["centres"], q_rand=out["rand"], sample=smp)
+    print("envmap", env.shape)
+
+
 def film_inputs(seed=13, w=131, h=7):
     """(H, W, 3) film spanning the RGBE/PPM edge cases: zeros, values below
     the 1e-32 cut, denormal-free tiny values, [0, 1], above 1, powers of
@@ -207,6 +246,9 @@ if __name__ == "__main__":
         film_fixture(_libs.reference())
     elif sys.argv[1:] == ["camera"]:
         camera_fixture(_libs.reference())
+    elif sys.argv[1:] == ["envmap"]:
+        envmap_fixture(_libs.reference())
     else:
         main()
         film_fixture(_libs.reference())
+        envmap_fixture(_libs.reference())
