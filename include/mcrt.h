@@ -128,6 +128,21 @@ MCRT_API int mcrt_materials_add(mcrt_scene* scene, size_t n, const int32_t* type
 /* Bulk replacement of material colours (albedo / emission) after finalize,
  * for optimisation loops; rgb holds 3 floats per material id. */
 MCRT_API int mcrt_material_set_rgb(mcrt_scene* scene, const float* rgb, size_t nmat);
+/* Scene::setEnvmap(filename) and setEnvmap(filename, width, height)
+ * (scene.h:44-45; EnvmapLoader, envmap.cpp:98-140): a latitude-longitude map
+ * of float RGB rows, top-down (row v: polar angle pi v from +y; column u:
+ * azimuth 2 pi u).  width = height = 0 reads the file's {int32 width, int32
+ * height} header; otherwise the file holds the floats alone.  The map lights
+ * the scene from infinity in every render: escaping paths see it, and with
+ * nee on each diffuse vertex samples it (map pdf and cosine lobe, MIS;
+ * demo_pt.cpp:421-499, 564-589).  Allowed before or after finalize; waits
+ * for the device.  A batch keeps the map its scenes had at creation:
+ * mcrt_batch_render fails with MCRT_ERR_STATE once a scene's map changed. */
+MCRT_API int mcrt_scene_set_envmap(mcrt_scene* scene, const char* path, int32_t width, int32_t height);
+/* The same from host memory: rgb holds width * height * 3 floats, rows
+ * top-down (width * height <= 2^28); rgb = NULL removes the map.  The
+ * luminance must sum to a positive finite value. */
+MCRT_API int mcrt_scene_set_envmap_rgb(mcrt_scene* scene, const float* rgb, int32_t width, int32_t height);
 /* Scene::add per triangle: v9 = n x (a, b, c), vn9 = n x (na, nb, nc) or
  * NULL for the face normal normalize(cross(a-c, b-c)) at every corner
  * (triangle.cpp:67-74), mat = material id per triangle. */

@@ -78,6 +78,8 @@ def lib():
         "mcrt_material_add": (I, [P, I, P, ctypes.c_float, ctypes.POINTER(ctypes.c_int32)]),
         "mcrt_materials_add": (I, [P, Z, P, P, P, ctypes.POINTER(ctypes.c_int32)]),
         "mcrt_material_set_rgb": (I, [P, P, Z]),
+        "mcrt_scene_set_envmap": (I, [P, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32]),
+        "mcrt_scene_set_envmap_rgb": (I, [P, P, ctypes.c_int32, ctypes.c_int32]),
         "mcrt_scene_add_triangles": (I, [P, P, P, P, Z]),
         "mcrt_scene_load_obj": (I, [P, ctypes.c_char_p, P, ctypes.c_int32, ctypes.POINTER(Z)]),
         "mcrt_scene_set_camera": (I, [P, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float]),
@@ -191,6 +193,20 @@ class Scene:
     def set_material_rgb(self, rgb):
         rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
         _check(lib().mcrt_material_set_rgb(self.h, rgb.ctypes.data, rgb.shape[0]))
+
+    def set_envmap(self, rgb=None, path=None, width=0, height=0):
+        """Scene::setEnvmap: an (H, W, 3) float array (rows top-down), or a
+        file (`path`; width = height = 0 reads its size header); neither
+        removes the map."""
+        if path is not None:
+            _check(lib().mcrt_scene_set_envmap(self.h, os.fsencode(path), int(width), int(height)))
+        elif rgb is None:
+            _check(lib().mcrt_scene_set_envmap_rgb(self.h, None, 0, 0))
+        else:
+            rgb = np.ascontiguousarray(rgb, np.float32)
+            if rgb.ndim != 3 or rgb.shape[2] != 3:
+                raise ValueError("environment map must be (H, W, 3)")
+            _check(lib().mcrt_scene_set_envmap_rgb(self.h, rgb.ctypes.data, rgb.shape[1], rgb.shape[0]))
 
     def add(self, tris, mat, normals=None):
         """Scene::add for n triangles: tris (n,3,3), mat int or (n,)."""

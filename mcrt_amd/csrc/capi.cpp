@@ -210,6 +210,26 @@ MCRT_API int mcrt_material_set_rgb(mcrt_scene* s, const float* rgb, size_t nmat)
     return MCRT_OK;
 }
 
+MCRT_API int mcrt_scene_set_envmap_rgb(mcrt_scene* s, const float* rgb, int32_t width, int32_t height)
+{
+    if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_set_envmap_rgb: NULL scene");
+    if (rgb && (width <= 0 || height <= 0 || (int64_t)width * height > (int64_t(1) << 28)))
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_set_envmap_rgb: size %d x %d out of range", width, height);
+    return set_envmap(s, rgb, width, height);
+}
+
+MCRT_API int mcrt_scene_set_envmap(mcrt_scene* s, const char* path, int32_t width, int32_t height)
+{
+    if (!s || !path) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_set_envmap: NULL argument");
+    if ((width == 0) != (height == 0) || width < 0 || height < 0)
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_set_envmap: width and height both 0 (file header) or both > 0");
+    std::vector<float> rgb;
+    int w = width, h = height;
+    const int rc = read_envmap(path, w, h, rgb);
+    if (rc) return rc;
+    return set_envmap(s, rgb.data(), w, h);
+}
+
 MCRT_API int mcrt_scene_add_triangles(mcrt_scene* s, const float* v9, const float* vn9, const int32_t* mat,
                                       size_t n)
 {
@@ -576,6 +596,10 @@ MCRT_API int mcrt_batch_render(mcrt_batch* b, void* stream)
     if (!b) return fail(MCRT_ERR_INVALID_ARG, "mcrt_batch_render: NULL batch");
     HIPCHK(hipSetDevice(b->ctx->device));
     hipStream_t st = (hipStream_t)stream;
+    for (size_t i = 0; i < b->scenes.size(); i++)
+        if (b->scenes[i]->env_gen != b->env_gen[i])
+            return fail(MCRT_ERR_STATE,
+                        "mcrt_batch_render: the environment map of scene %zu changed after mcrt_batch_create", i);
     std::lock_guard<std::mutex> g(b->res.mu);
     if (!(b->flags & MCRT_BATCH_GRAPH)) return batch_enqueue(b, st);
     if (!b->exec)

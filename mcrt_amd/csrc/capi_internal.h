@@ -87,7 +87,6 @@ struct StreamRes
     std::mutex mu;
     DevBuf spill;
     PTWork ptw;
-    int64_t pt_capacity = 0;
 };
 }  // namespace mcrt
 
@@ -120,6 +119,9 @@ struct mcrt_scene
     // device residency
     mcrt::DevScene dev;
     mcrt::DevBuf d_nodes, d_refs, d_shade, d_verts, d_mats, d_lights, d_hot;
+    // environment map (dev.env_*); env_gen counts replacements
+    mcrt::DevBuf d_env_px, d_env_cdf_u, d_env_cdf_v;
+    uint64_t env_gen = 0;
     // staging for host-pointer queries
     std::mutex q_mu;
     mcrt::DevBuf q_in, q_out;
@@ -140,6 +142,11 @@ DevCamera make_camera(const mcrt_scene* s);
 // uploads the scene.
 int finalize_scene(mcrt_scene* s, int threads, const char* cache);
 int upload_materials(mcrt_scene* s);
+// Environment map from w x h RGB floats (rows top-down); rgb = NULL removes
+// it.  Waits for the device before replacing the tables.
+int set_envmap(mcrt_scene* s, const float* rgb, int w, int h);
+// Reads an environment-map file; w = h = 0: the file has the size header.
+int read_envmap(const char* path, int& w, int& h, std::vector<float>& rgb);
 int run_query(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, uint8_t* occ, int mode, int flags,
               hipStream_t stream, uint64_t* counters);
 // The resources of `stream` in s's context (created on first use; live as
@@ -149,7 +156,7 @@ StreamRes* stream_res(mcrt_scene* s, hipStream_t stream);
 void apply_trace_tuning(mcrt_scene* s);
 // The scene's device view for a launch with r's spill stacks (caller holds r->mu).
 int launch_view(mcrt_scene* s, StreamRes* r, DevScene& view);
-int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch);
+int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch, int rs);
 // Renders into a device film (W*H*3 floats, may be NULL with adj) on
 // `stream` (asynchronous); with adj also the path-replay gradients.  The
 // caller holds r->mu.
@@ -174,6 +181,8 @@ struct mcrt_batch
     std::vector<float*> films;
     int flags = 0;
     int max_depth = 0;
+    int rs = 2;                      // next-event rays per record (4: a scene has an environment map)
+    std::vector<uint64_t> env_gen;   // each scene's environment generation at creation
     int64_t total_paths = 0;
     std::vector<mcrt::BatchChunk> chunks;
     mcrt::DevBuf d_tab, d_roots, d_seg;

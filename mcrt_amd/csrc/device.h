@@ -49,6 +49,13 @@ struct DevScene
     int64_t spill_lanes = 0;           // lanes the spill buffer covers = resident lanes of a launch
     int32_t spill_depth = 0;           // entries per lane beyond the LDS stack
     int32_t lds_depth = 16;            // LDS stack entries per lane (8, 12, 16 or 24)
+    // environment map (env_w == 0: none)
+    const float* env_px = nullptr;     // per pixel {r, g, b, luminance}, rows top-down
+    const float* env_cdf_u = nullptr;  // CDF over the columns (env_w)
+    const float* env_cdf_v = nullptr;  // CDF over the rows of each column (env_w x env_h, column-major)
+    int32_t env_w = 0, env_h = 0;
+    float env_norm = 0.0f;             // 1 / sum of the sin-warped luminance
+    int32_t pad1 = 0;
 };
 
 // The part of a scene a traversal reads (batched launches look it up per ray).

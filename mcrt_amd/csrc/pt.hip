@@ -12,12 +12,20 @@
 //     resolve            next-event contributions of vertex k-1 (MIS),
 //                        added before anything of vertex k, as in the
 //                        reference's accumulation order
-//     shade (k <= max)   emitter hit, next-event rays at diffuse
-//                        vertices, BSDF continuation, Russian roulette
+//     shade (k <= max)   emitter hit or escape to the environment,
+//                        next-event rays at diffuse vertices, BSDF
+//                        continuation, Russian roulette
 //   finish               NaN -> 0, clamp >= 0 per sample
 //   accumulate           per-pixel sum in sample order x (1/spp)
 // Every kernel is grid-strided over a fixed grid and reads its work count
 // from device memory, so the whole chunk is launch-only (graph-capturable).
+//
+// A diffuse vertex's next-event record holds rs rays: the BSDF-sampled ray
+// and the light-sampled shadow ray of the area lights, and, with an
+// environment map (rs = 4), the shadow rays of its two environment samples
+// (one drawn from the map's pdf, one from the cosine lobe; demo_pt.cpp:
+// 421-499, 564-589).  A ray a scene does not need has a zero direction,
+// which every box rejects.
 //
 // Batched rendering (BATCH = true): the paths of many scenes -- each with
 // its own BVH, camera, film size, spp and depth -- share the chunks.  A
@@ -56,11 +64,11 @@ struct PTState
     float4* T;           // throughput (xyz)
     float4* L;           // radiance (xyz)
     uint8_t* last_diff;  // previous vertex was diffuse
-    float4* rays[2];     // [P continuation | 2P next-event] x 2 float4 per ray
+    float4* rays[2];     // [P continuation | rs P next-event] x 2 float4 per ray
     uint32_t* act[2];    // path id per continuation ray
-    float4* rec[2];      // next-event records, 4 float4 each
+    float4* rec[2];      // next-event records, rec_words(rs) float4 each
     uint32_t* cnt[2];    // {continuation rays, next-event rays}
-    int4* hits;          // 3P
+    int4* hits;          // (1 + rs) P
     float4* samples;     // P clamped sample radiance
     float4* Ltot;        // adjoint: radiance of pass 1
     float4* delta;       // adjoint: dLoss/dL per path
@@ -69,8 +77,12 @@ struct PTState
     uint16_t* sid;       // batch: scene of each path
     uint64_t* pkey;      // batch: mix64(mix64(seed) ^ pixel) of each path
     uint64_t* skey;      // batch: sample << 8 of each path
-    uint16_t* rsid[2];   // batch: scene of each ray slot (3P)
+    uint16_t* rsid[2];   // batch: scene of each ray slot ((1 + rs) P)
 };
+
+// float4 words of a next-event record: {T, pdf_b}, {albedo, factor2},
+// {wb, path}, {wo, light}, and with the environment {e1}, {e2}.
+__host__ __device__ constexpr int rec_words(int rs) { return rs == 4 ? 6 : 4; }
 
 // One scene of a render: its device data, camera and parameters.
 struct ScenePT
@@ -99,7 +111,7 @@ struct PTArgs
     int64_t P;              // paths in this chunk
     int64_t pix0;           // single scene: first pixel of the chunk
     int32_t nseg;           // batch: segments of this chunk
-    int32_t pad;
+    int32_t rs;             // next-event rays per record: 2, or 4 with an environment map
     const Segment* seg;     // batch
     const ScenePT* tab;     // batch: per-scene table
     const SceneRoot* roots; // batch: per-scene traversal roots
@@ -275,15 +287,82 @@ __global__ void k_generate(PTState S, PTArgs a)
     }
 }
 
-// Next-event contributions of one diffuse vertex (demo_pt.cpp:591-643 with
-// Scene::lightPdfW scene.cpp:239-267 and lightRadiance :345-366).
+// Area-light terms of a next-event record (demo_pt.cpp:591-643 with
+// Scene::lightPdfW scene.cpp:239-267 and lightRadiance :345-366): the
+// BSDF-sampled ray hitting an emitter, MIS-weighted against the light pdf,
+// then the light sample if its shadow ray reaches the sampled light.
+// em1/em2, hot1/hot2 and the coefficients c1/c2 of the two emissions are
+// what the adjoint scatters.
+__device__ __forceinline__ V3f area_terms(const DevScene& D, const int4 h1, const int4 h2, V3f col, V3f wb, V3f wo,
+                                          float pdf_b, float factor2, int li, V3f local, int& em1, int& em2,
+                                          int& hot1, int& hot2, V3f& c1, V3f& c2)
+{
+    float pdf_l = 0.0f;
+    V3f le1 = v3f(0.0f, 0.0f, 0.0f);
+    if (h1.x >= 0)
+    {
+        const TriShade ts = D.shade[h1.x];
+        const DevMaterial mt = D.mats[ts.mat];
+        if (mt.type == MCRT_LGHT)
+        {
+            const V3f ns1 = shading_normal(ts, float_of((uint32_t)h1.z), float_of((uint32_t)h1.w));
+            const float cosl = dot(ns1, -wb);
+            if (!(cosl < 1e-6f))
+            {
+                const DevLight Lt = D.lights[ts.light];
+                const float pdfA = (1.0f / Lt.area) * Lt.pdf;
+                const float t1 = float_of((uint32_t)h1.y);
+                pdf_l = pdfA * (t1 * t1) / cosl;
+            }
+            if (!(dot(ns1, -wb) < 0))
+            {
+                le1 = ld3(mt.rgb);
+                em1 = h1.x;
+                hot1 = ts.hot;
+            }
+        }
+    }
+    const float mis_b = pdf_b / (pdf_b + pdf_l);
+    local = local + (col * le1) * mis_b;
+    c1 = col * mis_b;
+    V3f f = v3f(0.0f, 0.0f, 0.0f);
+    if (li >= 0)
+    {
+        V3f le2 = v3f(0.0f, 0.0f, 0.0f);
+        if (h2.x >= 0)
+        {
+            const TriShade ts = D.shade[h2.x];
+            const DevMaterial mt = D.mats[ts.mat];
+            if (mt.type == MCRT_LGHT)
+            {
+                const V3f ns2 = shading_normal(ts, float_of((uint32_t)h2.z), float_of((uint32_t)h2.w));
+                if (!(dot(ns2, -wo) < 0) && ts.light == li)
+                {
+                    le2 = ld3(mt.rgb);
+                    em2 = h2.x;
+                    hot2 = ts.hot;
+                }
+            }
+        }
+        const float ip = 1.0f / kPi;
+        f = v3f(ip, ip, ip) * col;  // LambertianBSDF::evaluate (bsdf.cpp:96-116)
+        local = local + (f * le2) * factor2;
+    }
+    c2 = f * factor2;
+    return local;
+}
+
+// Next-event contributions of one diffuse vertex: the environment terms
+// (demo_pt.cpp:564-589), each counted when its shadow ray escapes, then the
+// area-light terms, in the reference's order of accumulation.
 template <bool ADJ, bool BATCH>
 __global__ void k_resolve(PTArgs a, PTState S, int cur)
 {
     __shared__ float acc[ADJ ? 2 * kHotFloats : 1];
     if (ADJ) hot_clear(acc);
     const int64_t P = a.P;
-    const uint32_t nrec = S.cnt[cur][1] / 2;
+    const int rs = a.rs;
+    const uint32_t nrec = S.cnt[cur][1] / (uint32_t)rs;
     const float4* R = S.rec[cur];
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nrec; base += (int64_t)gridDim.x * blockDim.x)
     {
@@ -293,73 +372,35 @@ __global__ void k_resolve(PTArgs a, PTState S, int cur)
         V3f g1 = v3f(0.0f, 0.0f, 0.0f), g2 = g1;
         if (act)
         {
-            const float4 r0 = R[4 * m], r1 = R[4 * m + 1], r2 = R[4 * m + 2], r3 = R[4 * m + 3];
-            const V3f T = f3(r0), col = f3(r1), wb = f3(r2), wo = f3(r3);
-            const float pdf_b = r0.w, factor2 = r1.w;
+            const float4* Rm = R + rec_words(rs) * m;
+            const float4 r0 = Rm[0], r1 = Rm[1], r2 = Rm[2], r3 = Rm[3];
+            const V3f T = f3(r0), col = f3(r1);
             const uint32_t path = bits_of(r2.w);
-            const int li = (int)bits_of(r3.w);
             const DevScene& D = BATCH ? a.tab[S.sid[path]].D : a.one.D;
-            const int4 h1 = S.hits[P + 2 * m];
+            const int64_t h0 = P + rs * m;
             V3f local = v3f(0.0f, 0.0f, 0.0f);
-            float pdf_l = 0.0f;
-            V3f le1 = v3f(0.0f, 0.0f, 0.0f);
-            if (h1.x >= 0)
+            if (rs == 4)
             {
-                const TriShade ts = D.shade[h1.x];
-                const DevMaterial mt = D.mats[ts.mat];
-                if (mt.type == MCRT_LGHT)
-                {
-                    const V3f ns1 = shading_normal(ts, float_of((uint32_t)h1.z), float_of((uint32_t)h1.w));
-                    const float cosl = dot(ns1, -wb);
-                    if (!(cosl < 1e-6f))
-                    {
-                        const DevLight Lt = D.lights[ts.light];
-                        const float pdfA = (1.0f / Lt.area) * Lt.pdf;
-                        const float t1 = float_of((uint32_t)h1.y);
-                        pdf_l = pdfA * (t1 * t1) / cosl;
-                    }
-                    if (!(dot(ns1, -wb) < 0))
-                    {
-                        le1 = ld3(mt.rgb);
-                        em1 = h1.x;
-                        hot1 = ts.hot;
-                    }
-                }
+                const V3f z = v3f(0.0f, 0.0f, 0.0f);
+                const V3f e1 = S.hits[h0 + 2].x < 0 ? f3(Rm[4]) : z;
+                const V3f e2 = S.hits[h0 + 3].x < 0 ? f3(Rm[5]) : z;
+                local = local + (e1 + e2);
             }
-            const float mis_b = pdf_b / (pdf_b + pdf_l);
-            local = local + (col * le1) * mis_b;
-            V3f f = v3f(0.0f, 0.0f, 0.0f);
-            if (li >= 0)
+            if (D.nlights > 0)
             {
-                const int4 h2 = S.hits[P + 2 * m + 1];
-                V3f le2 = v3f(0.0f, 0.0f, 0.0f);
-                if (h2.x >= 0)
+                const int li = (int)bits_of(r3.w);
+                V3f c1, c2;
+                local = area_terms(D, S.hits[h0], li >= 0 ? S.hits[h0 + 1] : make_int4(-1, 0, 0, 0), col, f3(r2),
+                                   f3(r3), r0.w, r1.w, li, local, em1, em2, hot1, hot2, c1, c2);
+                if (ADJ)
                 {
-                    const TriShade ts = D.shade[h2.x];
-                    const DevMaterial mt = D.mats[ts.mat];
-                    if (mt.type == MCRT_LGHT)
-                    {
-                        const V3f ns2 = shading_normal(ts, float_of((uint32_t)h2.z), float_of((uint32_t)h2.w));
-                        if (!(dot(ns2, -wo) < 0) && ts.light == li)
-                        {
-                            le2 = ld3(mt.rgb);
-                            em2 = h2.x;
-                            hot2 = ts.hot;
-                        }
-                    }
+                    const V3f dl = f3(S.delta[path]);
+                    g1 = (dl * T) * c1;
+                    g2 = (dl * T) * c2;
                 }
-                const float ip = 1.0f / kPi;
-                f = v3f(ip, ip, ip) * col;  // LambertianBSDF::evaluate (bsdf.cpp:96-116)
-                local = local + (f * le2) * factor2;
             }
             V3f Lv = f3(S.L[path]) + T * local;
             S.L[path] = f4(Lv);
-            if (ADJ)
-            {
-                const V3f dl = f3(S.delta[path]);
-                g1 = (dl * T) * (col * mis_b);
-                g2 = (dl * T) * (f * factor2);
-            }
         }
         if (ADJ)
         {
@@ -368,6 +409,47 @@ __global__ void k_resolve(PTArgs a, PTState S, int cur)
         }
     }
     if (ADJ) hot_flush(a.one.D, acc, S.g_alb, S.g_em);
+}
+
+// Environment next-event terms of a diffuse vertex before visibility
+// (EnvmapSampler, demo_pt.cpp:421-499, with MIS :564-589): e1 from a sample
+// of the map's pdf, e2 from a sample of the cosine lobe in the frame
+// u = normalize((0.3, 0.4, 0.5) x n), v = n x u.  d1 / d2 are their shadow
+// rays' directions, zero (rejected by every box) where the sample lies
+// below the surface and the term is zero.
+__device__ __forceinline__ void env_terms(const DevScene& D, uint64_t key, V3f ns, V3f col, V3f& e1, V3f& e2,
+                                          V3f& d1, V3f& d2)
+{
+    const float ip = 1.0f / kPi;
+    const V3f fl = v3f(ip, ip, ip) * col;  // LambertianBSDF::evaluate
+    V3f Ld, lrad;
+    float lpdf;
+    env_sample(D, rng_u01(key, 8), rng_u01(key, 9), rng_u01(key, 10), rng_u01(key, 11), Ld, lrad, lpdf);
+    const float g1 = fmax_ref(0.0f, dot(Ld, ns));
+    if (g1 > 0)
+    {
+        const float epdf = env_pdf(D, env_pixel(D, Ld));
+        V3f irr = ((fl * g1) * (1.0f / lpdf)) * lrad;
+        if (!(irr.x == irr.x && irr.y == irr.y && irr.z == irr.z)) irr = v3f(0.0f, 0.0f, 0.0f);
+        e1 = irr * (epdf / (epdf + fmax_ref(dot(Ld, ns), kEpsCos) / kPi));
+        d1 = Ld;
+    }
+    const float x = rng_u01(key, 12) * kTwoPi;
+    const float r2 = rng_u01(key, 13);
+    const float r2s = sqrtf(r2);
+    const V3f uu = nrm(cross(v3f(0.3f, 0.4f, 0.5f), ns));
+    const V3f vv = cross(ns, uu);
+    float sx, cx;
+    own_sincos(x, &sx, &cx);
+    const V3f wo2 = nrm(((uu * (cx * r2s)) + (vv * (sx * r2s))) + ns * sqrtf(1.0f - r2));
+    const float g2 = fmax_ref(0.0f, dot(wo2, ns));
+    if (g2 > 0)
+    {
+        const int pix = env_pixel(D, wo2);
+        const float bpdf = fmax_ref(dot(wo2, ns), kEpsCos) / kPi;
+        e2 = (col * env_rad(D, pix)) * (bpdf / (bpdf + env_pdf(D, pix)));
+        d2 = wo2;
+    }
 }
 
 // One vertex of every live path (demo_pt.cpp:523-651).
@@ -389,10 +471,19 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
             if (j >= A) break;
             const uint32_t p = S.act[cur][j];
             const int4 h = S.hits[j];
-            if (h.x < 0) break;  // miss: no environment map
             const uint16_t sid = BATCH ? S.sid[p] : 0;
             const ScenePT& sp = BATCH ? a.tab[sid] : a.one;
             const DevScene& D = sp.D;
+            if (h.x < 0)
+            {
+                // escape: the passive environment term (demo_pt.cpp:528-540)
+                if (D.env_w > 0 && (!sp.nee || k == 0 || !S.last_diff[p]))
+                {
+                    const V3f Lv = f3(S.L[p]) + f3(S.T[p]) * env_rad(D, env_pixel(D, f3(rays[2 * j + 1])));
+                    S.L[p] = f4(Lv);
+                }
+                break;
+            }
             const float4 ra = rays[2 * j], rb = rays[2 * j + 1];
             const V3f o = f3(ra), d = f3(rb);
             const float t = float_of((uint32_t)h.y), u = float_of((uint32_t)h.z), v = float_of((uint32_t)h.w);
@@ -439,54 +530,66 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 const uint64_t sample = (uint64_t)(sp.sample_offset + p % sp.spp);
                 key = rng_key(sp.seed, (uint64_t)pixel, sample, (uint32_t)k);
             }
-            if (sp.nee && mt.type == MCRT_DIFF && D.nlights > 0)
+            if (sp.nee && mt.type == MCRT_DIFF && (D.nlights > 0 || D.env_w > 0))
             {
-                const V3f wb = lambert_dir(ns, rng_u01(key, 0), rng_u01(key, 1));
-                const float pdf_b = fmax_ref(dot(wb, ns), kEpsCos) / kPi;
-                // Scene::sampleLightArea (scene.cpp:298-343)
-                const int li = sample_light(D.lights, D.nlights, rng_u01(key, 2));
-                const DevLight Lt = D.lights[li];
-                const float* tv = D.verts + 9 * (int64_t)Lt.tri;
-                const float r0 = sqrtf(rng_u01(key, 3));
-                const float lu = 1.0f - r0;
-                const float lv = rng_u01(key, 4) * r0;
-                const V3f pl = (ld3(tv) * ((1.0f - lu) - lv) + ld3(tv + 3) * lu) + ld3(tv + 6) * lv;
-                const V3f nl = shading_normal(D.shade[Lt.tri], lu, lv);
-                const float pdfA = (1.0f / Lt.area) * Lt.pdf;
-                V3f wo = pl - phit;
-                const float dist2 = len2(wo);
-                wo = nrm(wo);
-                const float cosL = dot(nl, -wo);
-                float pdfW = pdfA * dist2 / fabsf(cosL);  // __pdfAtoW (geometry.cpp:222-225)
-                if (cosL <= 0) pdfW = 0;
-                float factor2 = 0.0f;
-                int li_rec = -1;
-                if (pdfW > 0)
-                {
-                    const float pdf_bp = fmax_ref(dot(wo, ns), kEpsCos) / kPi;
-                    const float mis_l = pdfW / (pdfW + pdf_bp);
-                    const float cosr = stdmax(dot(ns, wo), 0.0f);
-                    factor2 = cosr / pdfW * mis_l;
-                    li_rec = li;
-                }
-                const uint32_t slot = atomicAdd(&S.cnt[nxt][1], 2u);
-                const int64_t m = slot / 2;
-                float4* R = S.rec[nxt];
-                R[4 * m] = make_float4(T.x, T.y, T.z, pdf_b);
-                R[4 * m + 1] = make_float4(col.x, col.y, col.z, factor2);
-                R[4 * m + 2] = make_float4(wb.x, wb.y, wb.z, float_of(p));
-                R[4 * m + 3] = make_float4(wo.x, wo.y, wo.z, float_of((uint32_t)li_rec));
+                const int rs = a.rs;
+                const uint32_t slot = atomicAdd(&S.cnt[nxt][1], (uint32_t)rs);
+                const int64_t m = slot / (uint32_t)rs;
+                const int64_t r0i = a.P + rs * m;
+                float4* R = S.rec[nxt] + rec_words(rs) * m;
                 float4* nr = S.rays[nxt];
-                store_ray(nr, a.P + 2 * m, phit, wb, kEpsRay, __builtin_inff());
-                if (li_rec >= 0)
-                    store_ray(nr, a.P + 2 * m + 1, phit, wo, kEpsRay, __builtin_inff());
-                else  // zero direction: the slab test rejects every box (ray.cpp:77-79)
-                    store_ray(nr, a.P + 2 * m + 1, phit, v3f(0.0f, 0.0f, 0.0f), kEpsRay, __builtin_inff());
-                if (BATCH)
+                const V3f zero = v3f(0.0f, 0.0f, 0.0f);
+                V3f wb = zero, wo = zero;
+                float pdf_b = 1.0f, factor2 = 0.0f;
+                int li_rec = -1;
+                if (D.nlights > 0)
                 {
-                    S.rsid[nxt][a.P + 2 * m] = sid;
-                    S.rsid[nxt][a.P + 2 * m + 1] = sid;
+                    wb = lambert_dir(ns, rng_u01(key, 0), rng_u01(key, 1));
+                    pdf_b = fmax_ref(dot(wb, ns), kEpsCos) / kPi;
+                    // Scene::sampleLightArea (scene.cpp:298-343)
+                    const int li = sample_light(D.lights, D.nlights, rng_u01(key, 2));
+                    const DevLight Lt = D.lights[li];
+                    const float* tv = D.verts + 9 * (int64_t)Lt.tri;
+                    const float r0 = sqrtf(rng_u01(key, 3));
+                    const float lu = 1.0f - r0;
+                    const float lv = rng_u01(key, 4) * r0;
+                    const V3f pl = (ld3(tv) * ((1.0f - lu) - lv) + ld3(tv + 3) * lu) + ld3(tv + 6) * lv;
+                    const V3f nl = shading_normal(D.shade[Lt.tri], lu, lv);
+                    const float pdfA = (1.0f / Lt.area) * Lt.pdf;
+                    wo = pl - phit;
+                    const float dist2 = len2(wo);
+                    wo = nrm(wo);
+                    const float cosL = dot(nl, -wo);
+                    float pdfW = pdfA * dist2 / fabsf(cosL);  // __pdfAtoW (geometry.cpp:222-225)
+                    if (cosL <= 0) pdfW = 0;
+                    if (pdfW > 0)
+                    {
+                        const float pdf_bp = fmax_ref(dot(wo, ns), kEpsCos) / kPi;
+                        const float mis_l = pdfW / (pdfW + pdf_bp);
+                        const float cosr = stdmax(dot(ns, wo), 0.0f);
+                        factor2 = cosr / pdfW * mis_l;
+                        li_rec = li;
+                    }
                 }
+                R[0] = make_float4(T.x, T.y, T.z, pdf_b);
+                R[1] = make_float4(col.x, col.y, col.z, factor2);
+                R[2] = make_float4(wb.x, wb.y, wb.z, float_of(p));
+                R[3] = make_float4(wo.x, wo.y, wo.z, float_of((uint32_t)li_rec));
+                // a zero direction fails every slab test (ray.cpp:77-79): the
+                // rays a vertex does not need cost one root visit
+                store_ray(nr, r0i, phit, wb, kEpsRay, __builtin_inff());
+                store_ray(nr, r0i + 1, phit, li_rec >= 0 ? wo : zero, kEpsRay, __builtin_inff());
+                if (rs == 4)
+                {
+                    V3f e1 = zero, e2 = zero, d1 = zero, d2 = zero;
+                    if (D.env_w > 0) env_terms(D, key, ns, col, e1, e2, d1, d2);
+                    R[4] = f4(e1);
+                    R[5] = f4(e2);
+                    store_ray(nr, r0i + 2, phit, d1, kEpsRay, __builtin_inff());
+                    store_ray(nr, r0i + 3, phit, d2, kEpsRay, __builtin_inff());
+                }
+                if (BATCH)
+                    for (int i = 0; i < rs; i++) S.rsid[nxt][r0i + i] = sid;
             }
             // Continuation (Material::sample, demo_pt.cpp:645-650)
             const V3f wi = -d;
@@ -619,7 +722,7 @@ void bounce_loop(const DevScene& Dt, PTState& S, const PTArgs& a, int max_depth,
         TraceBatch tb;
         tb.roots = a.roots;
         tb.rsid = S.rsid[cur];
-        launch_k_trace<false, false, true, BATCH>(Dt, trace_blocks(Dt, 3 * a.P), st, S.rays[cur], (int64_t)0,
+        launch_k_trace<false, false, true, BATCH>(Dt, trace_blocks(Dt, (1 + a.rs) * a.P), st, S.rays[cur], (int64_t)0,
                                                   S.cnt[cur], a.P, S.hits, (uint8_t*)nullptr,
                                                   (unsigned long long*)nullptr, tb);
         hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(g), dim3(kTPB), 0, st, a, S, cur);
@@ -676,29 +779,25 @@ ScenePT scene_pt(mcrt_scene* s, const DevScene& D, const mcrt_render_params* prm
 }
 }  // namespace
 
-// Workspace for up to P paths (batch: plus the per-path / per-ray scene ids
-// and random-number keys).
-int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch)
+// Workspace for up to P paths with rs next-event rays per record (batch:
+// plus the per-path / per-ray scene ids and random-number keys).
+int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch, int rs)
 {
     PTWork& w = r->ptw;
     int rc;
-    const size_t n = (size_t)P;
-    if (r->pt_capacity < P)
+    const size_t n = (size_t)P, nr = (size_t)(1 + rs) * n;
+    if ((rc = w.T.ensure(n * 16))) return rc;
+    if ((rc = w.L.ensure(n * 16))) return rc;
+    if ((rc = w.flags.ensure(n))) return rc;
+    for (int i = 0; i < 2; i++)
     {
-        if ((rc = w.T.ensure(n * 16))) return rc;
-        if ((rc = w.L.ensure(n * 16))) return rc;
-        if ((rc = w.flags.ensure(n))) return rc;
-        for (int i = 0; i < 2; i++)
-        {
-            if ((rc = w.rays[i].ensure(3 * n * 32))) return rc;
-            if ((rc = w.act[i].ensure(n * 4))) return rc;
-            if ((rc = w.rec[i].ensure(n * 64))) return rc;
-        }
-        if ((rc = w.cnt.ensure(4 * sizeof(uint32_t)))) return rc;
-        if ((rc = w.hits.ensure(3 * n * 16))) return rc;
-        if ((rc = w.samples.ensure(n * 16))) return rc;
-        r->pt_capacity = P;
+        if ((rc = w.rays[i].ensure(nr * 32))) return rc;
+        if ((rc = w.act[i].ensure(n * 4))) return rc;
+        if ((rc = w.rec[i].ensure(n * 16 * rec_words(rs)))) return rc;
     }
+    if ((rc = w.cnt.ensure(4 * sizeof(uint32_t)))) return rc;
+    if ((rc = w.hits.ensure(nr * 16))) return rc;
+    if ((rc = w.samples.ensure(n * 16))) return rc;
     if (adjoint)
     {
         if ((rc = w.Ltot.ensure(n * 16))) return rc;
@@ -710,7 +809,7 @@ int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch)
         if ((rc = w.pkey.ensure(n * 8))) return rc;
         if ((rc = w.skey.ensure(n * 8))) return rc;
         for (int i = 0; i < 2; i++)
-            if ((rc = w.rsid[i].ensure(3 * n * 2))) return rc;
+            if ((rc = w.rsid[i].ensure(nr * 2))) return rc;
     }
     return MCRT_OK;
 }
@@ -727,7 +826,8 @@ int render_scene(mcrt_scene* s, StreamRes* r, const mcrt_render_params* prm, flo
     int64_t pix_per_chunk = s->pt_chunk_paths / spp;
     if (pix_per_chunk < 1) pix_per_chunk = 1;
     if (pix_per_chunk > npix) pix_per_chunk = npix;
-    if ((rc = pt_workspace(r, pix_per_chunk * spp, adj != nullptr, false))) return rc;
+    const int rs = D.env_w > 0 ? 4 : 2;
+    if ((rc = pt_workspace(r, pix_per_chunk * spp, adj != nullptr, false, rs))) return rc;
     PTState S = state_of(r, adj != nullptr);
     S.g_alb = adj ? adj->d_albedo : nullptr;
     S.g_em = adj ? adj->d_emission : nullptr;
@@ -740,7 +840,7 @@ int render_scene(mcrt_scene* s, StreamRes* r, const mcrt_render_params* prm, flo
     }
     PTArgs a;
     a.nseg = 0;
-    a.pad = 0;
+    a.rs = rs;
     a.seg = nullptr;
     a.tab = nullptr;
     a.roots = nullptr;
@@ -777,6 +877,8 @@ int batch_init(mcrt_batch* b)
     int64_t pmax = s0->pt_chunk_paths;
     int max_sb = 0;
     b->max_depth = 0;
+    b->rs = 2;
+    b->env_gen.clear();
     std::vector<ScenePT> tab(n);
     std::vector<SceneRoot> roots(n);
     for (size_t i = 0; i < n; i++)
@@ -788,6 +890,8 @@ int batch_init(mcrt_batch* b)
         roots[i].refs = s->dev.refs;
         pmax = std::max<int64_t>(pmax, b->params[i].spp);
         max_sb = std::max(max_sb, s->dev.stack_bound);
+        if (s->dev.env_w > 0) b->rs = 4;
+        b->env_gen.push_back(s->env_gen);
         b->max_depth = std::max(b->max_depth, b->params[i].max_depth);
     }
     // chunks of whole pixels, at most pmax paths each
@@ -830,7 +934,7 @@ int batch_init(mcrt_batch* b)
     if ((rc = b->d_seg.upload(segs.data(), segs.size() * sizeof(Segment)))) return rc;
     int64_t pcap = 0;
     for (const BatchChunk& k : b->chunks) pcap = std::max(pcap, k.P);
-    if ((rc = pt_workspace(&b->res, pcap, false, true))) return rc;
+    if ((rc = pt_workspace(&b->res, pcap, false, true, b->rs))) return rc;
     DevScene& Dt = b->trace_view;
     Dt = s0->dev;
     Dt.nodes = nullptr;
@@ -854,7 +958,7 @@ int batch_enqueue(mcrt_batch* b, hipStream_t st)
     const int cus = b->ctx->num_cus;
     PTArgs a;
     a.pix0 = 0;
-    a.pad = 0;
+    a.rs = b->rs;
     a.tab = b->d_tab.as<ScenePT>();
     a.roots = b->d_roots.as<SceneRoot>();
     a.one = ScenePT();

@@ -85,6 +85,61 @@ MCRT_HD float own_pow(float x, float y)
     if (!(x > 0.0f)) return 0.0f;
     return own_exp(y * own_log(x));
 }
+// atan (Cephes atanf: reduction at tan(3pi/8) and tan(pi/8), odd minimax
+// polynomial), asin (Cephes asinf: half-angle reduction above 1/2) and acos
+// from asin.
+MCRT_HD float own_atan(float x)
+{
+    const float sign = x < 0.0f ? -1.0f : 1.0f;
+    if (x < 0.0f) x = -x;
+    float y = 0.0f;
+    if (x > 2.414213562373095f)
+    {
+        y = 1.5707963267948966f;
+        x = -(1.0f / x);
+    }
+    else if (x > 0.4142135623730950f)
+    {
+        y = 0.7853981633974483f;
+        x = (x - 1.0f) / (x + 1.0f);
+    }
+    const float z = x * x;
+    y = y + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z * x +
+             x);
+    return sign * y;
+}
+MCRT_HD float own_asin(float xx)
+{
+    const float a = fabsf(xx);
+    if (a > 1.0f) return __builtin_nanf("");
+    if (a < 1.0e-4f) return xx;
+    float x, z;
+    const bool half = a > 0.5f;
+    if (half)
+    {
+        z = 0.5f * (1.0f - a);
+        x = sqrtf(z);
+    }
+    else
+    {
+        x = a;
+        z = x * x;
+    }
+    z = ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z +
+         1.6666752422e-1f) * z * x + x;
+    if (half)
+    {
+        z = z + z;
+        z = 1.5707963267948966f - z;
+    }
+    return xx < 0.0f ? -z : z;
+}
+MCRT_HD float own_acos(float x)
+{
+    if (x < -0.5f) return 3.14159265358979f - 2.0f * own_asin(sqrtf(0.5f * (1.0f + x)));
+    if (x > 0.5f) return 2.0f * own_asin(sqrtf(0.5f * (1.0f - x)));
+    return 1.5707963267948966f - own_asin(x);
+}
 
 // ------------------------------------------------------------ vec3 ------
 struct V3f
@@ -103,6 +158,10 @@ MCRT_HD V3f nrm(V3f a)  // vectors.cpp:227-231
 {
     float inv = 1.0f / sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
     return V3f{a.x * inv, a.y * inv, a.z * inv};
+}
+MCRT_HD V3f cross(V3f a, V3f b)  // vectors.cpp:180-185
+{
+    return V3f{(a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x)};
 }
 MCRT_HD float fmax_ref(float a, float b) { return a > b ? a : b; }   // f_max (numeric.cpp:27-30)
 MCRT_HD float stdmax(float a, float b) { return (a < b) ? b : a; }   // std::max
@@ -176,5 +235,71 @@ MCRT_HD int sample_light(const DevLight* L, int n, float r)
         else b = mid + 1;
     }
     return b;
+}
+MCRT_HD int sample_cdf(const float* cdf, int n, float r)
+{
+    int b = 0, e = n - 1;
+    while (e > b)
+    {
+        int mid = b + (e - b) / 2;
+        if (cdf[mid] >= r) e = mid;
+        else b = mid + 1;
+    }
+    return b;
+}
+
+// ------------------------------------------------ environment map -------
+// Latitude-longitude map (EnvmapLoader, envmap.cpp): row v <-> polar angle
+// phi = pi v from +y, column u <-> azimuth theta = 2 pi u, direction
+// (sin phi sin theta, cos phi, -sin phi cos theta).
+
+// static_cast<int>(x) clamped to [0, hi] (clampi, envmap.cpp:27-30); NaN and
+// negatives give 0.
+MCRT_HD int env_trunc(double x, int hi)
+{
+    if (!(x >= 0.0)) return 0;
+    if (x >= (double)hi) return hi;
+    return (int)x;
+}
+
+// Pixel of a direction (getPdf / sampleRadiance, envmap.cpp:440-484, with
+// dir_to_theta :44-53): theta = atan(z / x) + pi/2 (+ pi when x < 0), the
+// sum formed in double as the reference's double atan does.
+MCRT_HD int env_pixel(const DevScene& D, V3f d)
+{
+    const float phi = own_acos(d.y);
+    float theta = (float)((double)own_atan(d.z / d.x) + 1.5707963267948966);
+    if (d.x < 0) theta += kPi;
+    const int u = env_trunc((double)(theta / (kPi * 2.0f) * (float)D.env_w), D.env_w - 1);
+    const int v = env_trunc((double)(phi / kPi * (float)D.env_h), D.env_h - 1);
+    return u + v * D.env_w;
+}
+
+// Solid-angle pdf of a pixel: norm * lum * W * H / (2 pi^2); the sin(phi)
+// of the pre-warp cancels the pixel's solid angle (envmap.cpp:434-441).
+MCRT_HD float env_pdf(const DevScene& D, int pix)
+{
+    return (((D.env_norm * D.env_px[4 * (int64_t)pix + 3]) * (float)D.env_w) * (float)D.env_h) /
+           ((2.0f * kPi) * kPi);
+}
+MCRT_HD V3f env_rad(const DevScene& D, int pix) { return ld3(D.env_px + 4 * (int64_t)pix); }
+
+// EnvmapLoader::sample (envmap.cpp:332-456): column by the column-sum CDF,
+// row by that column's CDF, (j1, j2) the jitter inside the pixel.
+MCRT_HD void env_sample(const DevScene& D, float r1, float r2, float j1, float j2, V3f& dir, V3f& rad, float& pdf)
+{
+    const int W = D.env_w, H = D.env_h;
+    const int ui = sample_cdf(D.env_cdf_u, W, r1);
+    const int vi = sample_cdf(D.env_cdf_v + (int64_t)ui * H, H, r2);
+    const float u = ((float)ui + j1) / (float)W;
+    const float v = ((float)vi + j2) / (float)H;
+    const int pix = env_trunc((double)(u * (float)W), W - 1) + env_trunc((double)(v * (float)H), H - 1) * W;
+    const float phi = v * kPi, theta = (u * 2.0f) * kPi;
+    float sp, cp, st, ct;
+    own_sincos(phi, &sp, &cp);
+    own_sincos(theta, &st, &ct);
+    dir = v3f(sp * st, cp, sp * -ct);
+    pdf = env_pdf(D, pix);
+    rad = env_rad(D, pix);
 }
 }  // namespace mcrt

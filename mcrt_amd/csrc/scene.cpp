@@ -362,6 +362,125 @@ int finalize_scene(mcrt_scene* s, int threads, const char* cache)
     return MCRT_OK;
 }
 
+// EnvmapLoader::buildCdf (envmap.cpp:208-331) for a latitude-longitude
+// map of w x h RGB pixels, rows top-down: luminance 0.299 r + 0.587 g +
+// 0.114 b (numeric.cpp:62-66) per pixel, formed in double; a copy pre-warped
+// by sin(phi) of the row centre, phi = pi (m + 0.5) / h, multiplied in
+// double (the reference's unqualified sin of a float resolves to the double
+// function); norm = 1 / the in-order float sum of the warped values; the
+// DiscreteSampler CDFs (sample.cpp:103-118) of every column and of the
+// column sums.  Device layout: {r, g, b, luminance} per pixel (one 16 B
+// fetch gives radiance and pdf), the column CDFs column-major (each binary
+// search reads one contiguous run).
+int set_envmap(mcrt_scene* s, const float* rgb, int w, int h)
+{
+    DevScene& d = s->dev;
+    HIPCHK(hipSetDevice(s->ctx->device));
+    // renders in flight may read the current tables
+    HIPCHK(hipDeviceSynchronize());
+    if (!rgb)
+    {
+        s->env_gen++;
+        d.env_px = d.env_cdf_u = d.env_cdf_v = nullptr;
+        d.env_w = d.env_h = 0;
+        d.env_norm = 0.0f;
+        s->d_env_px.release();
+        s->d_env_cdf_u.release();
+        s->d_env_cdf_v.release();
+        return MCRT_OK;
+    }
+    const size_t n = (size_t)w * (size_t)h;
+    std::vector<float> px(4 * n), warped(n);
+    for (size_t i = 0; i < n; i++)
+    {
+        const float r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        px[4 * i] = r;
+        px[4 * i + 1] = g;
+        px[4 * i + 2] = b;
+        px[4 * i + 3] = (float)(0.299 * (double)r + 0.587 * (double)g + 0.114 * (double)b);
+    }
+    const float pi = 3.14159265358979323846f;
+    for (int m = 0; m < h; m++)
+    {
+        const float phi = pi * ((float)m + 0.5f) / (float)h;
+        const double sp = std::sin((double)phi);
+        for (int c = 0; c < w; c++)
+        {
+            const size_t i = (size_t)c + (size_t)m * w;
+            warped[i] = (float)((double)px[4 * i + 3] * sp);
+        }
+    }
+    float total = 0.0f;
+    for (float x : warped) total += x;
+    const float norm = 1.0f / total;
+    if (!(total > 0.0f) || !std::isfinite(total) || !std::isfinite(norm))
+        return fail(MCRT_ERR_INVALID_ARG, "environment map: the luminance must sum to a positive finite value");
+    auto cdf = [](const float* v, int k, float* out) {
+        float z = 0.0f;
+        for (int i = 0; i < k; i++) z += v[i];
+        const float inv = 1.0f / z;
+        float run = 0.0f;
+        for (int i = 0; i < k; i++)
+        {
+            run += v[i] * inv;
+            out[i] = run;
+        }
+    };
+    std::vector<float> col(h), colsum(w), cdf_v(n), cdf_u(w);
+    for (int c = 0; c < w; c++)
+    {
+        float acc = 0.0f;
+        for (int m = 0; m < h; m++)
+        {
+            col[m] = warped[(size_t)c + (size_t)m * w];
+            acc += col[m];
+        }
+        colsum[c] = acc;
+        cdf(col.data(), h, &cdf_v[(size_t)c * h]);
+    }
+    cdf(colsum.data(), w, cdf_u.data());
+    s->env_gen++;
+    int rc;
+    if ((rc = s->d_env_px.upload(px.data(), px.size() * sizeof(float)))) return rc;
+    if ((rc = s->d_env_cdf_u.upload(cdf_u.data(), cdf_u.size() * sizeof(float)))) return rc;
+    if ((rc = s->d_env_cdf_v.upload(cdf_v.data(), cdf_v.size() * sizeof(float)))) return rc;
+    d.env_px = s->d_env_px.as<float>();
+    d.env_cdf_u = s->d_env_cdf_u.as<float>();
+    d.env_cdf_v = s->d_env_cdf_v.as<float>();
+    d.env_w = w;
+    d.env_h = h;
+    d.env_norm = norm;
+    return MCRT_OK;
+}
+
+// The radiance of an environment-map file (EnvmapLoader's constructors,
+// envmap.cpp:98-140): {int32 width, int32 height} then the floats, or, with
+// width and height given, the floats alone.
+int read_envmap(const char* path, int& w, int& h, std::vector<float>& rgb)
+{
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(MCRT_ERR_IO, "cannot open environment map %s", path);
+    int32_t hd[2] = {w, h};
+    if (w == 0 && h == 0 && std::fread(hd, sizeof(int32_t), 2, f) != 2)
+    {
+        std::fclose(f);
+        return fail(MCRT_ERR_IO, "%s: no environment-map header", path);
+    }
+    if (hd[0] <= 0 || hd[1] <= 0 || (int64_t)hd[0] * hd[1] > (int64_t(1) << 28))
+    {
+        std::fclose(f);
+        return fail(MCRT_ERR_INVALID_ARG, "%s: environment map size %d x %d out of range", path, hd[0], hd[1]);
+    }
+    w = hd[0];
+    h = hd[1];
+    const size_t n = (size_t)w * h * 3;
+    rgb.resize(n);
+    const size_t got = std::fread(rgb.data(), sizeof(float), n, f);
+    std::fclose(f);
+    if (got != n) return fail(MCRT_ERR_IO, "%s: %zu of %zu floats", path, got, n);
+    return MCRT_OK;
+}
+
 void apply_trace_tuning(mcrt_scene* s)
 {
     DevScene& d = s->dev;

@@ -267,6 +267,44 @@ def inverse_rendering(freq=71, hidden_seed=5):
     return target, init_rgb
 
 
+def outdoor(sub=3, light=True, phong_frac=0.3, phong_exp=16.0, albedo_seed=3, emission=8.0):
+    """An open scene for environment lighting, framed by CAMERA: a ground
+    quad (y = 0, facing up), a displaced icosphere with per-triangle albedo
+    (a fraction glossy) at the Cornell sphere's place, a small flat-shaded
+    ball beside it and, with `light`, a 20 x 20 area light facing down
+    above them.  The sky fills the top half of the image."""
+    ground = np.array([[[-100, 0, -100], [-100, 0, 200], [150, 0, 200]],
+                       [[-100, 0, -100], [150, 0, 200], [150, 0, -100]]], np.float32)
+    v, f = icosphere(sub)
+    sv = BOX_CENTER + SPHERE_RADIUS * v * (1.0 + 0.05 * value_noise3(v, seed=7))[:, None]
+    sph, sph_n = sv[f].astype(np.float32), vertex_normals(sv, f)[f].astype(np.float32)
+    bv, bf = icosphere(1)
+    ball = (np.array([5.0, 6.0, 10.0]) + 6.0 * bv)[bf].astype(np.float32)
+    parts = [ground, sph, ball]
+    normals = [np.repeat(_face_normals(ground)[:, None], 3, 1), sph_n, np.repeat(_face_normals(ball)[:, None], 3, 1)]
+    ns = len(f)
+    rng = np.random.default_rng(albedo_seed)
+    alb = rng.uniform(0.2, 0.9, (ns, 3))
+    phong = rng.uniform(0.0, 1.0, ns) < phong_frac
+    # materials: 0 light, 1 ground, 2 ball, 3.. sphere triangles
+    mat_type = np.concatenate([[api.LGHT, api.DIFF, api.DIFF], np.where(phong, api.GLSY, api.DIFF)]).astype(np.int32)
+    mat_rgb = np.concatenate([[(emission,) * 3, (0.5, 0.45, 0.4), (0.7, 0.7, 0.7)], alb])
+    tri_mat = [np.full(2, 1, np.int32), 3 + np.arange(ns, dtype=np.int32), np.full(len(bf), 2, np.int32)]
+    if light:
+        lq = np.array([[[18, 60, 18], [38, 60, 38], [18, 60, 38]], [[18, 60, 18], [38, 60, 18], [38, 60, 38]]],
+                      np.float32)
+        parts.insert(0, lq)
+        normals.insert(0, np.repeat(_face_normals(lq)[:, None], 3, 1))
+        tri_mat.insert(0, np.zeros(2, np.int32))
+    tris = np.concatenate(parts)
+    fn = _face_normals(tris)
+    assert fn[-len(bf) - ns - 1, 1] > 0.99  # ground faces up
+    if light:
+        assert fn[0, 1] < -0.99 and fn[1, 1] < -0.99  # light faces down
+    gloss = np.where(mat_type == api.GLSY, phong_exp, 25.0)
+    return SceneData(tris, np.concatenate(normals), np.concatenate(tri_mat), mat_type, mat_rgb, gloss)
+
+
 def sky_envmap(width=128, height=64, seed=5, sun=(0.35, 0.3), sun_radiance=60.0):
     """Synthetic latitude-longitude environment (rows top-down: row v <->
     polar angle pi v from +y, column u <-> azimuth 2 pi u): a blue-to-white

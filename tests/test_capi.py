@@ -42,6 +42,8 @@ def test_errors_without_gpu_or_with_null_arguments():
     assert L.mcrt_scene_destroy(None) == 1
     assert L.mcrt_intersect(None, None, 0, None, 0, None) == 1
     assert L.mcrt_render(None, None, None, 0, None) == 1
+    assert L.mcrt_scene_set_envmap(None, b"x.bin", 0, 0) == 1
+    assert L.mcrt_scene_set_envmap_rgb(None, None, 0, 0) == 1
     try:
         import torch
         has_gpu = torch.cuda.is_available()
