@@ -120,6 +120,59 @@ def render_leg(scene, data, steps, warm, cpu):
     return out
 
 
+ENV = dict(width=1024, height=1024, spp=16, max_depth=8, rr_start=3, rr_prob=0.8, seed=5, env_w=512, env_h=256)
+
+
+def env_leg(ctx, steps, cpu):
+    """Msamples/s with environment lighting: the displaced icosphere (sub 8,
+    per-triangle albedo, 30% glossy) over a ground quad with an area light,
+    under a 512 x 256 sky with a sun; four next-event rays per diffuse
+    vertex.  The oracle's fp32 integrator renders a band of rows."""
+    import torch
+    from mcrt_amd import scenes
+    E = ENV
+    data = scenes.outdoor(sub=8)
+    env = scenes.sky_envmap(E["env_w"], E["env_h"], seed=5)
+    s = data.to_scene(ctx, width=E["width"], height=E["height"])
+    s.set_envmap(env)
+    kw = {k: E[k] for k in ("spp", "max_depth", "rr_start", "rr_prob", "seed")}
+    film = torch.zeros((E["height"], E["width"], 3), dtype=torch.float32, device="cuda")
+    s.render(film, **kw)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(torch.cuda.current_stream())
+    for _ in range(steps):
+        s.render(film, **kw)
+    e1.record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nsamp = E["width"] * E["height"] * E["spp"]
+    out = {"msamples_per_s": round(nsamp / ms / 1e3, 2), "ms_per_frame": round(ms, 2),
+           "config": dict(E, scene="outdoor_ico8_sky", triangles=data.ntri, nee=True),
+           "film_mean": round(float(film.mean()), 6)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _libs
+        osc = _libs.OracleScene(data)
+        osc.set_envmap(env)
+        cores = _libs.oracle().orc_num_procs()
+        rows = (E["height"] // 2 - 8, E["height"] // 2 + 8)
+        t0 = time.time()
+        cfilm = osc.render(E["width"], E["height"], E["spp"], E["max_depth"], E["rr_start"], E["rr_prob"],
+                           E["seed"], rows=rows, threads=cores)
+        dt = time.time() - t0
+        cs = (rows[1] - rows[0]) * E["width"] * E["spp"]
+        band = film[rows[0]:rows[1]].cpu().numpy()
+        out["cpu_baseline"] = {"value": round(cs / dt / 1e6, 4), "unit": "Msamples/s", "cores": int(cores),
+                               "kind": "port", "sample": f"rows {rows[0]}..{rows[1]} ({cs} samples)",
+                               "band_bit_identical_to_gpu":
+                                   bool(np.array_equal(band.view(np.uint32), cfilm[rows[0]:rows[1]].view(np.uint32)))}
+        out["gpu_over_cpu"] = round(out["msamples_per_s"] / out["cpu_baseline"]["value"], 1)
+    s.close()
+    return out
+
+
 ADJOINT = dict(width=512, height=512, spp=32, max_depth=6, rr_start=3, rr_prob=0.8, seed=7)
 
 
@@ -276,6 +329,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the 4M-triangle soup ray-query leg")
     ap.add_argument("--no-c4", action="store_true", help="skip the 64-scene batched rendering leg")
+    ap.add_argument("--no-env", action="store_true", help="skip the environment-lighting render leg")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -381,6 +435,8 @@ def main():
         out["c3_soup"] = c3_leg(ctx, 5)
     if not (args.no_c4 or args.profile):
         out["c4_batch"] = c4_leg(ctx, 2)
+    if not (args.no_env or args.profile):
+        out["env_render"] = env_leg(ctx, 2, cpu=not args.no_cpu)
     if not args.no_adjoint:
         a = adjoint_leg(ctx, 1 if args.profile else 3)
         out["adjoint"] = a

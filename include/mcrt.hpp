@@ -3,7 +3,8 @@
  * reference's scene / film interface so that code written against it
  * changes little when it moves to the GPU path:
  *   mcrt::Scene        Scene (src/core/scene.h:28-103): add, loadModel,
- *                      setCamera, finalize, intersect, occluded, numTrangles,
+ *                      setEnvmap, setCamera, finalize, intersect, occluded,
+ *                      numTrangles,
  *                      plus render() for the demo_pt render loop
  *                      (src/examples/pt/demo_pt.cpp:657-720)
  *   mcrt::FrameBuffer  FrameBuffer (src/core/film.h:6-31): resize, width,
@@ -193,6 +194,18 @@ public:
         size_t n = 0;
         check(mcrt_scene_load_obj(h_, path.c_str(), m16, material, &n));
         return n;
+    }
+    /* Scene::setEnvmap(filename) (file with its size header) and
+     * setEnvmap(filename, width, height) (floats only), scene.h:44-45. */
+    void setEnvmap(const std::string& path) { check(mcrt_scene_set_envmap(h_, path.c_str(), 0, 0)); }
+    void setEnvmap(const std::string& path, int width, int height)
+    {
+        check(mcrt_scene_set_envmap(h_, path.c_str(), width, height));
+    }
+    /* From memory: width * height * 3 floats, rows top-down; NULL removes it. */
+    void setEnvmap(const float* rgb, int width, int height)
+    {
+        check(mcrt_scene_set_envmap_rgb(h_, rgb, width, height));
     }
     void setCamera(const Vec3& o, const Vec3& target, int width, int height, float fov_y)
     {

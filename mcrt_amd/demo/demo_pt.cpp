@@ -4,13 +4,14 @@
 //
 //   demo_pt [--width W] [--height H] [--spp N] [--depth D] [--rr-start K]
 //           [--rr-prob P] [--seed S] [--obj FILE] [--albedo A]
-//           [--icosphere SUB] [--out PREFIX] [--raw FILE]
+//           [--icosphere SUB] [--envmap FILE] [--out PREFIX] [--raw FILE]
 //
 // Scene: the Cornell box at the reference's x0.1 scale (grey 0.75 floor,
 // back wall and ceiling, red and green side walls) with its 2-triangle
 // ceiling light (emission 15), plus either the OBJ file (one diffuse
 // material of albedo A, Scene::loadModel) or a centred icosphere of
-// subdivision SUB built here.  Writes PREFIX.hdr (linear film) and
+// subdivision SUB built here; --envmap adds an environment map
+// (Scene::setEnvmap, as demo_photonmap.cpp:289 does).  Writes PREFIX.hdr (linear film) and
 // PREFIX.ppm (after tonemapGamma(2.2)) like the reference's test_pt.hdr /
 // test_pt.ppm, and with --raw the float film (H x W x 3, row 0 = bottom).
 #include <algorithm>
@@ -36,7 +37,7 @@ struct Args
     int width = 256, height = 256, spp = 16, depth = 8, rr_start = 3, icosphere = 4;
     float rr_prob = 0.8f, albedo = 0.8f;
     unsigned long long seed = 1;
-    std::string obj, out = "test_pt", raw;
+    std::string obj, out = "test_pt", raw, envmap;
 };
 
 bool parse(int argc, char** argv, Args& a)
@@ -58,6 +59,7 @@ bool parse(int argc, char** argv, Args& a)
         else if (k == "--icosphere") a.icosphere = std::atoi(v);
         else if (k == "--out") a.out = v;
         else if (k == "--raw") a.raw = v;
+        else if (k == "--envmap") a.envmap = v;
         else return false;
     }
     return true;
@@ -156,7 +158,7 @@ int main(int argc, char** argv)
     {
         std::fprintf(stderr, "usage: %s [--width W] [--height H] [--spp N] [--depth D] [--rr-start K] "
                              "[--rr-prob P] [--seed S] [--obj FILE] [--albedo A] [--icosphere SUB] "
-                             "[--out PREFIX] [--raw FILE]\n", argv[0]);
+                             "[--envmap FILE] [--out PREFIX] [--raw FILE]\n", argv[0]);
         return 2;
     }
     try
@@ -174,6 +176,11 @@ int main(int argc, char** argv)
         }
         else
             icosphere(scene, a.icosphere, a.albedo);
+        if (!a.envmap.empty())
+        {
+            scene.setEnvmap(a.envmap);
+            std::printf("loaded envmap %s\n", a.envmap.c_str());
+        }
         const mcrt_build_stats st = scene.finalize();
         std::printf("init complete: %zu triangles, SBVH %.3f s, %lld refs\n", scene.numTrangles(),
                     st.build_seconds, (long long)st.refs);

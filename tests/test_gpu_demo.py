@@ -15,7 +15,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DEMO = os.path.join(ROOT, "mcrt_amd", "bin", "demo_pt")
 
 
-def test_demo_pt_matches_c_abi_render(ctx, tmp_path):
+@pytest.mark.parametrize("with_env", [False, True])
+def test_demo_pt_matches_c_abi_render(ctx, tmp_path, with_env):
     import mcrt_amd
     from mcrt_amd import scenes
     if not os.path.exists(DEMO):
@@ -30,6 +31,10 @@ def test_demo_pt_matches_c_abi_render(ctx, tmp_path):
     cmd = [DEMO, "--width", str(W), "--height", str(H), "--spp", str(spp), "--depth", str(depth),
            "--rr-start", "3", "--rr-prob", "0.8", "--seed", "1", "--obj", str(obj), "--albedo", "0.8",
            "--out", str(out), "--raw", str(raw)]
+    env = scenes.sky_envmap(64, 32, seed=3) if with_env else None
+    if with_env:
+        scenes.write_envmap(tmp_path / "sky.bin", env)
+        cmd += ["--envmap", str(tmp_path / "sky.bin")]
     print("[demo]", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     print(r.stdout, flush=True)
@@ -49,6 +54,8 @@ def test_demo_pt_matches_c_abi_render(ctx, tmp_path):
     s.add(box, np.array([mat[k] for k in kinds], np.int32))
     sm = s.add_material(mcrt_amd.DIFF, (0.8, 0.8, 0.8))
     assert s.load_model(obj, sm) == sphere.shape[0]
+    if with_env:
+        s.set_envmap(env)
     s.finalize()
     film = s.render(spp=spp, max_depth=depth, rr_start=3, rr_prob=0.8, seed=1)
     assert np.array_equal(demo.view(np.uint32), film.view(np.uint32))
