@@ -241,21 +241,27 @@ MCRT_API int mcrt_film_tonemap_gamma(float* film, size_t n, float gamma);
  * flight. */
 typedef struct mcrt_tuning {
     int32_t trace_waves_per_cu; /* traversal workgroups (one wavefront each) per CU,
-                                   1..1024 (256); beyond what fits at once the
-                                   hardware scheduler balances the ragged tail      */
+                                   1..1024 (32); the waves claim rays from work
+                                   counters, so more than fit at once only adds
+                                   launch overhead                                   */
     int32_t trace_lds_depth;    /* traversal stack entries per lane kept in LDS: 8, 12,
                                    16 or 24 (8); deeper entries go to a per-stream
                                    global spill area                                    */
     int64_t pt_chunk_paths;     /* paths per wavefront chunk of the renders, 1..2^28
                                    (2^24); path state is ~0.5 KB per path              */
+    int32_t pt_trace_waves_per_cu; /* the same two settings for the traversal
+                                      launches of renders, whose ray sets (many
+                                      short shadow rays) favour more waves: (32) */
+    int32_t pt_trace_lds_depth;    /* (8)                                         */
 } mcrt_tuning;
 MCRT_API int mcrt_scene_set_tuning(mcrt_scene* scene, const mcrt_tuning* tuning);
 MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* scene, mcrt_tuning* tuning);
 
 /* Autotuner (no reference counterpart): times closest-hit over the n rays
- * for trace_waves_per_cu in {32,64,128,256,512} x trace_lds_depth in
+ * for trace_waves_per_cu in {8,16,24,32,64,128} x trace_lds_depth in
  * {8,12,16} and keeps the fastest; with `render` != NULL also times one
- * render per pt_chunk_paths in 2^20..2^24 and keeps the fastest.  The
+ * render per pt_trace_waves_per_cu in {16,32,64} x pt_trace_lds_depth in
+ * {8,16}, then per pt_chunk_paths in 2^20..2^24, keeping the fastest.  The
  * chosen tuning is applied to the scene and returned in `best`, the best
  * closest-hit rate in `best_mrays_per_s` (either may be NULL).  Rays follow
  * `flags` like the queries; synchronous. */

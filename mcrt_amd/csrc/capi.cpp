@@ -432,6 +432,10 @@ static int check_tuning(const mcrt_tuning* t)
         return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_lds_depth must be 8, 12, 16 or 24 (0 = keep)");
     if (t->pt_chunk_paths < 0 || t->pt_chunk_paths > (int64_t(1) << 28))
         return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: pt_chunk_paths in [1, 2^28] (0 = keep)");
+    if (t->pt_trace_waves_per_cu < 0 || t->pt_trace_waves_per_cu > 1024)
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: pt_trace_waves_per_cu in [1, 1024] (0 = keep)");
+    if (t->pt_trace_lds_depth != 0 && !valid_lds_depth(t->pt_trace_lds_depth))
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: pt_trace_lds_depth must be 8, 12, 16 or 24 (0 = keep)");
     return MCRT_OK;
 }
 
@@ -443,6 +447,8 @@ MCRT_API int mcrt_scene_set_tuning(mcrt_scene* s, const mcrt_tuning* t)
     if (t->trace_waves_per_cu) s->trace_waves_per_cu = t->trace_waves_per_cu;
     if (t->trace_lds_depth) s->trace_lds_depth = t->trace_lds_depth;
     if (t->pt_chunk_paths) s->pt_chunk_paths = t->pt_chunk_paths;
+    if (t->pt_trace_waves_per_cu) s->pt_trace_waves_per_cu = t->pt_trace_waves_per_cu;
+    if (t->pt_trace_lds_depth) s->pt_trace_lds_depth = t->pt_trace_lds_depth;
     if (s->finalized) apply_trace_tuning(s);
     return MCRT_OK;
 }
@@ -453,6 +459,8 @@ MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* s, mcrt_tuning* t)
     t->trace_waves_per_cu = s->trace_waves_per_cu;
     t->trace_lds_depth = s->trace_lds_depth;
     t->pt_chunk_paths = s->pt_chunk_paths;
+    t->pt_trace_waves_per_cu = s->pt_trace_waves_per_cu;
+    t->pt_trace_lds_depth = s->pt_trace_lds_depth;
     return MCRT_OK;
 }
 
@@ -489,7 +497,7 @@ MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcr
             r = d_rays.as<float>();
         }
         if ((rc = d_hits.ensure(n * sizeof(mcrt_hit)))) return rc;
-        static const int kWaves[] = {32, 64, 128, 256, 512};
+        static const int kWaves[] = {8, 16, 24, 32, 64, 128};
         static const int kDepth[] = {8, 12, 16};
         int bw = s->trace_waves_per_cu, bd = s->trace_lds_depth;
         for (int w : kWaves)
@@ -520,26 +528,52 @@ MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcr
         s->trace_lds_depth = bd;
         apply_trace_tuning(s);
     }
-    // Path tracer: paths per wavefront chunk, one render of the camera each.
+    // Path tracer: its traversal launches (waves per CU x LDS depth), then the
+    // paths per wavefront chunk; one timed render of the camera each.
     if (render)
     {
         const int64_t total = (int64_t)s->dev_cam.w * s->dev_cam.h * render->spp;
         DevBuf film;
         if ((rc = film.ensure((size_t)s->dev_cam.w * s->dev_cam.h * 3 * sizeof(float)))) return rc;
-        int64_t bc = s->pt_chunk_paths;
+        auto time_render = [&](float& ms) -> int {
+            int e = render_scene(s, res, render, film.as<float>(), nullptr, st);
+            if (e) return e;
+            HIPCHK(hipEventRecord(e0, st));
+            if ((e = render_scene(s, res, render, film.as<float>(), nullptr, st))) return e;
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            return MCRT_OK;
+        };
+        static const int kPtWaves[] = {16, 32, 64};
+        static const int kPtDepth[] = {8, 16};
+        int pw = s->pt_trace_waves_per_cu, pd = s->pt_trace_lds_depth;
         float best_ms = 0.0f;
+        for (int w : kPtWaves)
+            for (int d : kPtDepth)
+            {
+                s->pt_trace_waves_per_cu = w;
+                s->pt_trace_lds_depth = d;
+                float ms = 0.0f;
+                if ((rc = time_render(ms))) return rc;
+                if (best_ms == 0.0f || ms < best_ms)
+                {
+                    best_ms = ms;
+                    pw = w;
+                    pd = d;
+                }
+            }
+        s->pt_trace_waves_per_cu = pw;
+        s->pt_trace_lds_depth = pd;
+        int64_t bc = s->pt_chunk_paths;
+        best_ms = 0.0f;
         for (int lg = 20; lg <= 24; lg++)
         {
             const int64_t c = int64_t(1) << lg;
             if (lg > 20 && (c >> 1) >= total) break;  // larger chunks change nothing
             s->pt_chunk_paths = c;
-            if ((rc = render_scene(s, res, render, film.as<float>(), nullptr, st))) return rc;
-            HIPCHK(hipEventRecord(e0, st));
-            if ((rc = render_scene(s, res, render, film.as<float>(), nullptr, st))) return rc;
-            HIPCHK(hipEventRecord(e1, st));
-            HIPCHK(hipEventSynchronize(e1));
             float ms = 0.0f;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if ((rc = time_render(ms))) return rc;
             if (best_ms == 0.0f || ms < best_ms)
             {
                 best_ms = ms;

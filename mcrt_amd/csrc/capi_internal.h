@@ -86,6 +86,7 @@ struct StreamRes
 {
     std::mutex mu;
     DevBuf spill;
+    DevBuf wq;  // ray-query work counter
     PTWork ptw;
 };
 }  // namespace mcrt
@@ -127,11 +128,14 @@ struct mcrt_scene
     mcrt::DevBuf q_in, q_out;
     // launch tuning (mcrt_scene_set_tuning / mcrt_autotune)
     // defaults measured on MI355X (1.3M-triangle displaced icosphere, 16M
-    // incoherent rays): 8-entry LDS stacks let 32 waves (62 VGPRs each) be
-    // resident per CU, and 256 workgroups per CU -- 8 times what fits --
-    // leave the hardware scheduler to balance the ragged tail of the lanes
-    int32_t trace_waves_per_cu = 256;
+    // incoherent rays; 1024^2 renders of it and of an outdoor scene): waves
+    // claim rays from work counters, so the grid only has to fill the
+    // machine once: 32 one-wave workgroups per CU with 8-entry LDS stacks
+    // (8 waves per SIMD) were fastest for queries and renders alike
+    int32_t trace_waves_per_cu = 32;
     int32_t trace_lds_depth = 8;
+    int32_t pt_trace_waves_per_cu = 32;
+    int32_t pt_trace_lds_depth = 8;
     int64_t pt_chunk_paths = int64_t(1) << 24;
 };
 
@@ -156,6 +160,10 @@ StreamRes* stream_res(mcrt_scene* s, hipStream_t stream);
 void apply_trace_tuning(mcrt_scene* s);
 // The scene's device view for a launch with r's spill stacks (caller holds r->mu).
 int launch_view(mcrt_scene* s, StreamRes* r, DevScene& view);
+// The same for the traversal launches of renders (pt_trace_* tuning).
+int render_view(mcrt_scene* s, StreamRes* r, DevScene& view);
+// Sets the traversal geometry of `view` (LDS depth, spill lanes and depth).
+void trace_geometry(DevScene& view, int num_cus, int waves_per_cu, int lds_depth);
 int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch, int rs);
 // Renders into a device film (W*H*3 floats, may be NULL with adj) on
 // `stream` (asynchronous); with adj also the path-replay gradients.  The

@@ -49,6 +49,7 @@ struct DevScene
     int64_t spill_lanes = 0;           // lanes the spill buffer covers = resident lanes of a launch
     int32_t spill_depth = 0;           // entries per lane beyond the LDS stack
     int32_t lds_depth = 16;            // LDS stack entries per lane (8, 12, 16 or 24)
+    uint32_t* wq = nullptr;            // ray-query work counters, kWqWords words (per stream)
     // environment map (env_w == 0: none)
     const float* env_px = nullptr;     // per pixel {r, g, b, luminance}, rows top-down
     const float* env_cdf_u = nullptr;  // CDF over the columns (env_w)
@@ -64,6 +65,12 @@ struct SceneRoot
     const QNode* nodes = nullptr;
     const WRef* refs = nullptr;
 };
+
+// Work counters of a traversal launch: one per slice of the rays, 64 B
+// apart (trace_kernel.h).
+constexpr uint32_t kSlices = 8;
+constexpr uint32_t kSliceStride = 16;  // uint32 words between slice counters
+constexpr uint32_t kWqWords = kSlices * kSliceStride;
 
 // Traversal modes.
 enum TraceMode

@@ -67,7 +67,8 @@ struct PTState
     float4* rays[2];     // [P continuation | rs P next-event] x 2 float4 per ray
     uint32_t* act[2];    // path id per continuation ray
     float4* rec[2];      // next-event records, rec_words(rs) float4 each
-    uint32_t* cnt[2];    // {continuation rays, next-event rays}
+    uint32_t* cnt[2];    // {continuation rays, next-event rays}, then the traversal's work
+                         // counters from word kCntWq on (kSideWords words per side)
     int4* hits;          // (1 + rs) P
     float4* samples;     // P clamped sample radiance
     float4* Ltot;        // adjoint: radiance of pass 1
@@ -204,8 +205,12 @@ __device__ __forceinline__ void grad_add(float* acc, float* g, int key, int hot,
     scatter3(g, key, v, valid && hot < 0);
 }
 
+// Counts of one side, and its traversal work counters zeroed.
+constexpr int kCntWq = 16;
+constexpr int kSideWords = kCntWq + (int)kWqWords;
 __global__ void k_set_counts(uint32_t* cnt, uint32_t a, uint32_t b)
 {
+    if (blockIdx.x == 0 && threadIdx.x < kSlices) cnt[kCntWq + threadIdx.x * kSliceStride] = 0;
     if (threadIdx.x == 0 && blockIdx.x == 0)
     {
         cnt[0] = a;
@@ -724,7 +729,7 @@ void bounce_loop(const DevScene& Dt, PTState& S, const PTArgs& a, int max_depth,
         tb.rsid = S.rsid[cur];
         launch_k_trace<false, false, true, BATCH>(Dt, trace_blocks(Dt, (1 + a.rs) * a.P), st, S.rays[cur], (int64_t)0,
                                                   S.cnt[cur], a.P, S.hits, (uint8_t*)nullptr,
-                                                  (unsigned long long*)nullptr, tb);
+                                                  (unsigned long long*)nullptr, S.cnt[cur] + kCntWq, tb);
         hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(g), dim3(kTPB), 0, st, a, S, cur);
         if (k <= max_depth)
         {
@@ -747,7 +752,7 @@ PTState state_of(StreamRes* r, bool adjoint)
         S.rays[i] = w.rays[i].as<float4>();
         S.act[i] = w.act[i].as<uint32_t>();
         S.rec[i] = w.rec[i].as<float4>();
-        S.cnt[i] = w.cnt.as<uint32_t>() + 2 * i;
+        S.cnt[i] = w.cnt.as<uint32_t>() + kSideWords * i;
         S.rsid[i] = w.rsid[i].as<uint16_t>();
     }
     S.hits = w.hits.as<int4>();
@@ -795,7 +800,7 @@ int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch, int rs)
         if ((rc = w.act[i].ensure(n * 4))) return rc;
         if ((rc = w.rec[i].ensure(n * 16 * rec_words(rs)))) return rc;
     }
-    if ((rc = w.cnt.ensure(4 * sizeof(uint32_t)))) return rc;
+    if ((rc = w.cnt.ensure(2 * kSideWords * sizeof(uint32_t)))) return rc;
     if ((rc = w.hits.ensure(nr * 16))) return rc;
     if ((rc = w.samples.ensure(n * 16))) return rc;
     if (adjoint)
@@ -818,7 +823,7 @@ int render_scene(mcrt_scene* s, StreamRes* r, const mcrt_render_params* prm, flo
                  hipStream_t st)
 {
     DevScene D;
-    int rc = launch_view(s, r, D);
+    int rc = render_view(s, r, D);
     if (rc) return rc;
     const DevCamera& cam = s->dev_cam;
     const int64_t npix = (int64_t)cam.w * cam.h;
@@ -939,10 +944,8 @@ int batch_init(mcrt_batch* b)
     Dt = s0->dev;
     Dt.nodes = nullptr;
     Dt.refs = nullptr;
-    Dt.lds_depth = s0->trace_lds_depth;
-    Dt.spill_lanes = (int64_t)b->ctx->num_cus * s0->trace_waves_per_cu * 64;
-    Dt.spill_depth = max_sb > Dt.lds_depth ? max_sb - Dt.lds_depth : 0;
-    Dt.spill = nullptr;
+    Dt.stack_bound = max_sb;
+    trace_geometry(Dt, b->ctx->num_cus, s0->pt_trace_waves_per_cu, s0->pt_trace_lds_depth);
     if (Dt.spill_depth > 0)
     {
         if ((rc = b->res.spill.ensure((size_t)Dt.spill_lanes * Dt.spill_depth * sizeof(uint64_t)))) return rc;

@@ -481,13 +481,17 @@ int read_envmap(const char* path, int& w, int& h, std::vector<float>& rgb)
     return MCRT_OK;
 }
 
-void apply_trace_tuning(mcrt_scene* s)
+void trace_geometry(DevScene& d, int num_cus, int waves_per_cu, int lds_depth)
 {
-    DevScene& d = s->dev;
-    d.lds_depth = s->trace_lds_depth;
-    d.spill_lanes = (int64_t)s->ctx->num_cus * s->trace_waves_per_cu * 64;
+    d.lds_depth = lds_depth;
+    d.spill_lanes = (int64_t)num_cus * waves_per_cu * 64;
     d.spill_depth = d.stack_bound > d.lds_depth ? d.stack_bound - d.lds_depth : 0;
     d.spill = nullptr;  // per stream, see launch_view
+}
+
+void apply_trace_tuning(mcrt_scene* s)
+{
+    trace_geometry(s->dev, s->ctx->num_cus, s->trace_waves_per_cu, s->trace_lds_depth);
 }
 
 StreamRes* stream_res(mcrt_scene* s, hipStream_t stream)
@@ -496,6 +500,19 @@ StreamRes* stream_res(mcrt_scene* s, hipStream_t stream)
     std::unique_ptr<StreamRes>& r = s->ctx->res[stream];
     if (!r) r.reset(new StreamRes());
     return r.get();
+}
+
+int render_view(mcrt_scene* s, StreamRes* r, DevScene& view)
+{
+    int rc = launch_view(s, r, view);
+    if (rc) return rc;
+    trace_geometry(view, s->ctx->num_cus, s->pt_trace_waves_per_cu, s->pt_trace_lds_depth);
+    if (view.spill_depth > 0)
+    {
+        if ((rc = r->spill.ensure((size_t)view.spill_lanes * view.spill_depth * sizeof(uint64_t)))) return rc;
+        view.spill = r->spill.as<uint64_t>();
+    }
+    return MCRT_OK;
 }
 
 int launch_view(mcrt_scene* s, StreamRes* r, DevScene& view)
@@ -507,6 +524,9 @@ int launch_view(mcrt_scene* s, StreamRes* r, DevScene& view)
         if (rc) return rc;
         view.spill = r->spill.as<uint64_t>();
     }
+    int rc = r->wq.ensure(kWqWords * sizeof(uint32_t));
+    if (rc) return rc;
+    view.wq = r->wq.as<uint32_t>();
     return MCRT_OK;
 }
 

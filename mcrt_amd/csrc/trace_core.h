@@ -30,6 +30,7 @@ namespace mcrt
 {
 MCRT_HD uint32_t bits_of(float f) { return __builtin_bit_cast(uint32_t, f); }
 MCRT_HD float float_of(uint32_t u) { return __builtin_bit_cast(float, u); }
+MCRT_HD float4 f4_of(uint4 u) { return make_float4(float_of(u.x), float_of(u.y), float_of(u.z), float_of(u.w)); }
 
 struct RayPrep
 {
@@ -194,16 +195,34 @@ struct Tracer
         sp = 0;
     }
 
-    // Visits `cur`; returns true once the traversal has ended.  Sc is any
+    // The 64 B record `cur` names: a QNode, or a leaf reference.  Sc is any
     // type with the scene's `nodes` and `refs` (DevScene, SceneRoot).
     template <class Sc>
+    MCRT_HD const uint4* record(const Sc& S) const
+    {
+        return (cur & MCRT_CHILD_LEAF) ? reinterpret_cast<const uint4*>(S.refs + (cur & MCRT_LEAF_REF_MASK))
+                                       : reinterpret_cast<const uint4*>(S.nodes + cur);
+    }
+
+    // Visits `cur`; returns true once the traversal has ended.  Node and
+    // leaf records are both 64 B, so one fetch serves every lane of a
+    // wavefront whichever kind each lane visits: a step costs one memory
+    // round trip.
+    template <class Sc>
     MCRT_HD bool step(const Sc& S, TraceStack& st, TraceCount& cnt)
+    {
+        const uint4* p = record(S);
+        return visit(p[0], p[1], p[2], p[3], st, cnt);
+    }
+
+    // Visits `cur` given its record w0..w3.
+    MCRT_HD bool visit(const uint4 w0, const uint4 w1, const uint4 w2, const uint4 w3, TraceStack& st,
+                       TraceCount& cnt)
     {
         if (!(cur & MCRT_CHILD_LEAF))
         {
             // QNode (layout.h): four 16 B rows, child planes origin + q * 2^k
-            const uint4* p = reinterpret_cast<const uint4*>(S.nodes + cur);
-            const uint4 hd = p[0], ch = p[1], qa = p[2], qb = p[3];
+            const uint4 hd = w0, ch = w1, qa = w2, qb = w3;
             const float ox = float_of(hd.x), oy = float_of(hd.y), oz = float_of(hd.z);
             const float sx = float_of((hd.w & 0xffu) << 23);
             const float sy = float_of(((hd.w >> 8) & 0xffu) << 23);
@@ -258,8 +277,7 @@ struct Tracer
             // `cur` and are tested in slot order by the following steps.
             const uint32_t first = cur & MCRT_LEAF_REF_MASK;
             const uint32_t count = ((cur >> 29) & 3u) + 1u;
-            const float4* q = reinterpret_cast<const float4*>(S.refs + first);
-            const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+            const float4 a = f4_of(w0), b = f4_of(w1), c = f4_of(w2), d = f4_of(w3);
             if (COUNT) cnt.refs++;
             float lo;
             if (slab(r, a.x, a.w, a.y, b.x, a.z, b.y, R1, lo))

@@ -1,8 +1,20 @@
 // The traversal kernel template, shared by the query entry points
 // (traverse.hip) and the path tracer's extend step (pt.hip).
 //
-// One lane per ray at a time, grid-strided over the batch, one wavefront
-// per workgroup (its LDS stack slab is private to the wave).  Two ray sources:
+// One lane per ray at a time, one wavefront per workgroup (its LDS stack
+// slab is private to the wave).  Waves claim rays kClaim at a time from a
+// counter in device memory (`wq`, zero at launch): a lane whose ray ends
+// takes the next index of its wave's claim, and the wave claims again when
+// the claim is used up, so every lane stays busy until the batch runs out
+// instead of idling once its share of a static split is done.  The rays
+// are split into kSlices contiguous slices, one counter each (64 B apart),
+// and a wave starts on slice blockIdx % kSlices, i.e. on its XCD's slice as
+// workgroups are dealt to the XCDs in turn; a wave whose slice is used up
+// moves on to the next.  Device-scope atomics on one address serialise, so
+// spreading them over eight counters and claiming kClaim rays (4 per lane)
+// at a time keeps the counters off the critical path (the path tracer's
+// launches hold up to 5 rays per path, most of them short).  Two ray
+// sources:
 //   * SEG = false: rays[0, n) with n known to the host;
 //   * SEG = true : two segments whose sizes live in device memory (the
 //     path tracer's compacted continuation rays [0, cnt[0]) and its
@@ -16,6 +28,7 @@
 namespace mcrt
 {
 constexpr int kWave = 64;
+constexpr uint32_t kClaim = 256;
 
 // Rays of several scenes in one launch: ray i belongs to scene rsid[i].
 struct TraceBatch
@@ -24,11 +37,15 @@ struct TraceBatch
     const uint16_t* rsid = nullptr;
 };
 
+// With an 8-entry stack, eight waves per SIMD (32 per CU) fit in LDS; they
+// also need at most 64 VGPRs and 96 SGPRs, which the attribute asks for.
+// Deeper stacks are held to fewer waves by LDS.
 template <int D, bool ANY, bool COUNT, bool SEG, bool BATCH>
-__global__ __launch_bounds__(kWave) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 8 : 4, 8))) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
                                                  const uint32_t* __restrict__ seg_cnt, int64_t seg1,
                                                  int4* __restrict__ hits, uint8_t* __restrict__ occ,
-                                                 unsigned long long* __restrict__ counters, TraceBatch B)
+                                                 unsigned long long* __restrict__ counters, TraceBatch B,
+                                                 uint32_t* __restrict__ wq)
 {
     __shared__ uint32_t s_node[D * kWave];
     __shared__ float s_lo[D * kWave];
@@ -41,8 +58,9 @@ __global__ __launch_bounds__(kWave) void k_trace(DevScene S, const float4* __res
     st.spill = S.spill;
     st.spill_lane = blockIdx.x * kWave + threadIdx.x;
     st.spill_depth = S.spill_depth;
-    // 32-bit ray indices: a launch covers fewer than 2^31 rays (launch_trace
-    // splits larger batches; path-tracer chunks are at most 3 * 2^28 rays).
+    // 32-bit ray indices: a launch covers at most 2^31 rays (launch_trace
+    // splits larger batches; path-tracer chunks are at most 5 * 2^28 rays),
+    // and a slice counter ends below its slice + 2 * kClaim * (waves of the grid).
     uint32_t n0 = (uint32_t)n, total = (uint32_t)n;
     if (SEG)
     {
@@ -51,42 +69,80 @@ __global__ __launch_bounds__(kWave) void k_trace(DevScene S, const float4* __res
     }
     TraceCount cnt;
     uint32_t nrays = 0;
-    const uint32_t stride = gridDim.x * kWave;
-    uint32_t k = blockIdx.x * kWave + threadIdx.x;
-    // Lane-persistent loop: each lane walks its rays k, k + stride, ... and
-    // starts the next one in the same iteration in which the previous ends,
-    // so divergent ray lengths do not idle the wavefront.
+    const uint32_t lane = threadIdx.x;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64u - lane));  // lanes under this one
     Tracer<ANY, COUNT> tr;
     SceneRoot root;
     root.nodes = S.nodes;
     root.refs = S.refs;
     uint32_t i = 0;
-    bool live = k < total;
-    if (live)
+    uint32_t claim = 0, claim_end = 0;  // wave-uniform: indices [claim, claim_end) not yet started
+    bool drained = false;               // wave-uniform: every slice is used up
+    bool live = false;
+    // slice s holds the indices [total * s / kSlices, total * (s + 1) / kSlices)
+    auto slice_lo = [total](uint32_t s) { return (uint32_t)(((uint64_t)total * s) / kSlices); };
+    uint32_t sl = blockIdx.x % kSlices, tried = 0;  // wave-uniform: current slice, slices found used up
+    // The next window is requested when the current one is taken, and its
+    // offset is read (lane 0's atomic result) only when that one runs out,
+    // so the atomic's round trip overlaps traversal steps instead of
+    // stalling the wave between short rays.
+    uint32_t ahead = 0;  // lane 0: offset of the requested next window in slice sl
+    if (lane == 0) ahead = atomicAdd(wq + sl * kSliceStride, kClaim);
+    for (;;)
     {
-        i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : k;
-        if (BATCH) root = B.roots[B.rsid[i]];
-        const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
-        tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w);
-    }
-    while (live)
-    {
-        if (root.nodes != nullptr && !tr.step(root, st, cnt)) continue;
-        float t, u, v;
-        const int prim = tr.result(t, u, v);
-        if (ANY)
-            occ[i] = prim >= 0 ? 1 : 0;
-        else
-            hits[i] = make_int4(prim, (int)bits_of(t), (int)bits_of(u), (int)bits_of(v));
-        nrays++;
-        k += stride;
-        live = k < total;
-        if (live)
+        // idle lanes take the next indices of the claim, in lane order
+        uint64_t idle = __ballot(!live);
+        while (idle != 0 && !drained)
         {
-            i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : k;
-            if (BATCH) root = B.roots[B.rsid[i]];
-            const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
-            tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w);
+            if (claim == claim_end)
+            {
+                uint32_t c = __shfl(ahead, 0);
+                uint32_t lo = slice_lo(sl), len = slice_lo(sl + 1) - lo;
+                while (c >= len)  // slice used up: move on to the next one
+                {
+                    if (++tried == kSlices)
+                    {
+                        drained = true;
+                        break;
+                    }
+                    sl = (sl + 1) % kSlices;
+                    lo = slice_lo(sl);
+                    len = slice_lo(sl + 1) - lo;
+                    uint32_t t = 0;
+                    if (lane == 0) t = atomicAdd(wq + sl * kSliceStride, kClaim);
+                    c = __shfl(t, 0);
+                }
+                if (drained) break;
+                if (lane == 0) ahead = atomicAdd(wq + sl * kSliceStride, kClaim);
+                claim = lo + c;
+                claim_end = len - c < kClaim ? lo + len : lo + c + kClaim;
+            }
+            const uint32_t avail = claim_end - claim;
+            const uint32_t rank = (uint32_t)__popcll(idle & below);
+            if (!live && rank < avail)
+            {
+                const uint32_t k = claim + rank;
+                i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : k;
+                if (BATCH) root = B.roots[B.rsid[i]];
+                const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+                tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w);
+                live = true;
+            }
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            claim += nidle < avail ? nidle : avail;
+            idle = __ballot(!live);
+        }
+        if (__ballot(live) == 0) break;  // drained and every lane done: the whole wave exits
+        if (live && (root.nodes == nullptr || tr.step(root, st, cnt)))
+        {
+            float t, u, v;
+            const int prim = tr.result(t, u, v);
+            if (ANY)
+                occ[i] = prim >= 0 ? 1 : 0;
+            else
+                hits[i] = make_int4(prim, (int)bits_of(t), (int)bits_of(u), (int)bits_of(v));
+            nrays++;
+            live = false;
         }
     }
     if (COUNT)
@@ -109,30 +165,31 @@ inline int64_t trace_blocks(const DevScene& s, int64_t n)
     return blocks;
 }
 
-// Launches k_trace with the scene's LDS stack depth on `blocks` workgroups.
+// Launches k_trace with the scene's LDS stack depth on `blocks` workgroups;
+// wq holds kWqWords words, the slice counters zero.
 template <bool ANY, bool COUNT, bool SEG, bool BATCH = false>
 inline void launch_k_trace(const DevScene& S, int64_t blocks, hipStream_t st, const float4* rays, int64_t n,
                            const uint32_t* seg_cnt, int64_t seg1, int4* hits, uint8_t* occ,
-                           unsigned long long* counters, TraceBatch B = TraceBatch())
+                           unsigned long long* counters, uint32_t* wq, TraceBatch B = TraceBatch())
 {
     const dim3 g((unsigned)blocks), b(kWave);
     switch (S.lds_depth)
     {
     case 8:
         hipLaunchKernelGGL((k_trace<8, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
-                           counters, B);
+                           counters, B, wq);
         break;
     case 12:
         hipLaunchKernelGGL((k_trace<12, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
-                           counters, B);
+                           counters, B, wq);
         break;
     case 24:
         hipLaunchKernelGGL((k_trace<24, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
-                           counters, B);
+                           counters, B, wq);
         break;
     default:
         hipLaunchKernelGGL((k_trace<16, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
-                           counters, B);
+                           counters, B, wq);
         break;
     }
 }

@@ -5,13 +5,20 @@ namespace mcrt
 {
 namespace
 {
+__global__ void k_wq_reset(uint32_t* wq)
+{
+    if (threadIdx.x < kSlices) wq[threadIdx.x * kSliceStride] = 0;
+}
+
 template <bool ANY, bool COUNT>
 hipError_t launch(const DevScene& s, const float* rays, int64_t n, void* hits, uint8_t* occ, uint64_t* counters,
                   hipStream_t stream)
 {
+    // a kernel, not hipMemsetAsync, so a caller may capture the query in a graph
+    hipLaunchKernelGGL(k_wq_reset, dim3(1), dim3(64), 0, stream, s.wq);
     launch_k_trace<ANY, COUNT, false>(s, trace_blocks(s, n), stream, reinterpret_cast<const float4*>(rays), n,
                                       nullptr, (int64_t)0, reinterpret_cast<int4*>(hits), occ,
-                                      reinterpret_cast<unsigned long long*>(counters));
+                                      reinterpret_cast<unsigned long long*>(counters), s.wq);
     return hipGetLastError();
 }
 }  // namespace

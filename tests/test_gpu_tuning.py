@@ -47,9 +47,11 @@ def test_render_invariant_under_tuning(ctx):
     s = d.to_scene(ctx, width=40, height=30)
     kw = dict(spp=8, max_depth=6, rr_start=2, rr_prob=0.8, seed=5)
     ref = s.render(**kw)
-    for waves, depth, chunk in ((4, 8, 1000), (32, 24, 1 << 20), (16, 12, 777)):
+    for waves, depth, chunk in ((4, 8, 1000), (32, 24, 1 << 20), (16, 12, 777), (1, 16, 5000)):
         print(f"[render tuning] waves={waves} depth={depth} chunk={chunk}", flush=True)
-        s.set_tuning(trace_waves_per_cu=waves, trace_lds_depth=depth, pt_chunk_paths=chunk)
+        s.set_tuning(pt_trace_waves_per_cu=waves, pt_trace_lds_depth=depth, pt_chunk_paths=chunk)
+        t = s.tuning()
+        assert (t["pt_trace_waves_per_cu"], t["pt_trace_lds_depth"], t["pt_chunk_paths"]) == (waves, depth, chunk)
         f = s.render(**kw)
         assert np.array_equal(f.view(np.uint32), ref.view(np.uint32))
 
@@ -66,8 +68,9 @@ def test_autotune_applies_valid_tuning(ctx):
     print(f"[autotune] {t} {rate:.1f} Mrays/s", flush=True)
     assert t == s.tuning()
     assert t["trace_lds_depth"] in (8, 12, 16)
-    assert t["trace_waves_per_cu"] in (32, 64, 128, 256, 512)
+    assert t["trace_waves_per_cu"] in (8, 16, 24, 32, 64, 128)
     assert t["pt_chunk_paths"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
+    assert t["pt_trace_waves_per_cu"] in (16, 32, 64) and t["pt_trace_lds_depth"] in (8, 16)
     assert rate > 0
     assert np.array_equal(s.intersect(rays), before)
     assert np.array_equal(s.render(spp=4, max_depth=4, seed=2).view(np.uint32), film0.view(np.uint32))
@@ -77,7 +80,7 @@ def test_tuning_errors(ctx):
     import mcrt_amd
     _, s, _ = _debug_scene(ctx)
     for kw in (dict(trace_lds_depth=10), dict(trace_waves_per_cu=1025), dict(trace_waves_per_cu=-1),
-               dict(pt_chunk_paths=(1 << 28) + 1)):
+               dict(pt_chunk_paths=(1 << 28) + 1), dict(pt_trace_lds_depth=4), dict(pt_trace_waves_per_cu=2000)):
         with pytest.raises(mcrt_amd.McrtError) as e:
             s.set_tuning(**kw)
         assert e.value.code == 1
