@@ -11,7 +11,7 @@
 // and a wave starts on slice blockIdx % kSlices, i.e. on its XCD's slice as
 // workgroups are dealt to the XCDs in turn; a wave whose slice is used up
 // moves on to the next.  Device-scope atomics on one address serialise, so
-// spreading them over eight counters and claiming kClaim rays (4 per lane)
+// spreading them over eight counters and claiming kClaim rays (2 per lane)
 // at a time keeps the counters off the critical path (the path tracer's
 // launches hold up to 5 rays per path, most of them short).  Two ray
 // sources:
@@ -28,7 +28,7 @@
 namespace mcrt
 {
 constexpr int kWave = 64;
-constexpr uint32_t kClaim = 256;
+constexpr uint32_t kClaim = 128;
 
 // Rays of several scenes in one launch: ray i belongs to scene rsid[i].
 struct TraceBatch
