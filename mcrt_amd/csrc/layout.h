@@ -43,13 +43,17 @@ static_assert(sizeof(WNode) == 128, "WNode must be 128 bytes");
 //          | child count << 24
 //   row 1: child[4] (links as in WNode)
 //   row 2: lo_x, hi_x, lo_y, hi_y   (byte c of each word = child c)
-//   row 3: lo_z, hi_z, order_lo, order_hi
+//   row 3: lo_z, hi_z, rank_lo, rank_hi
 // A plane decodes as origin + (float)q * 2^(e - 127) (q * 2^k is exact, one
 // rounding in the addition); the host chooses q so that every decoded box
 // contains the exact child box.  A contained box's slab interval is
 // contained after rounding, so a quantised test passes whenever the exact
 // one does; the extra subtrees it admits hold only references whose exact
-// clipped boxes fail, so results are those of the exact boxes.
+// clipped boxes fail, so results are those of the exact boxes.  The visit
+// order is stored inverted: byte o of rank_lo/rank_hi holds, 2 bits per
+// slot, each slot's position in octant o's visit order (the inverse of
+// WNode::order), so a kernel places the passing children on its stack
+// with fixed-slot arithmetic instead of indexing by slot.
 struct alignas(64) QNode
 {
     float origin[3];
@@ -57,8 +61,21 @@ struct alignas(64) QNode
     uint32_t child[4];
     uint32_t q_lo_x, q_hi_x, q_lo_y, q_hi_y;
     uint32_t q_lo_z, q_hi_z;
-    uint32_t order_lo, order_hi;
+    uint32_t rank_lo, rank_hi;
 };
+
+// The rank table of a WNode order table (either 32-bit half).
+inline uint32_t order_to_rank(uint32_t order)
+{
+    uint32_t rank = 0;
+    for (int o = 0; o < 4; o++)
+        for (int k = 0; k < 4; k++)
+        {
+            const uint32_t s = (order >> (8 * o + 2 * k)) & 3u;
+            rank |= (uint32_t)k << (8 * o + 2 * s);
+        }
+    return rank;
+}
 static_assert(sizeof(QNode) == 64, "QNode must be 64 bytes");
 
 // Decoded plane of a QNode (the kernels evaluate the same expression).

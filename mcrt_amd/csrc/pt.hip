@@ -125,6 +125,23 @@ __device__ __forceinline__ void store_ray(float4* rays, int64_t i, V3f o, V3f d,
     rays[2 * i + 1] = make_float4(d.x, d.y, d.z, tmax);
 }
 __device__ __forceinline__ V3f f3(float4 v) { return V3f{v.x, v.y, v.z}; }
+
+// Slot of this lane in a compaction by the lanes active at the call: one
+// atomic per wavefront (its lowest active lane adds per x the active count)
+// instead of one per lane -- device-scope atomics on one address
+// serialise -- and consecutive slots for consecutive lanes, so the writes
+// that follow coalesce.  Slot order never affects a result: every path's
+// records are its own.
+__device__ __forceinline__ uint32_t wave_slot(uint32_t* counter, uint32_t per)
+{
+    const uint64_t active = __ballot(1);
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(active) * per);
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(active & ((1ull << lane) - 1ull)) * per;
+}
 __device__ __forceinline__ float4 f4(V3f v) { return make_float4(v.x, v.y, v.z, 0.0f); }
 
 // Gradient scatter of the adjoint kernels.  Triangles with a hot slot
@@ -538,7 +555,7 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
             if (sp.nee && mt.type == MCRT_DIFF && (D.nlights > 0 || D.env_w > 0))
             {
                 const int rs = a.rs;
-                const uint32_t slot = atomicAdd(&S.cnt[nxt][1], (uint32_t)rs);
+                const uint32_t slot = wave_slot(&S.cnt[nxt][1], (uint32_t)rs);
                 const int64_t m = slot / (uint32_t)rs;
                 const int64_t r0i = a.P + rs * m;
                 float4* R = S.rec[nxt] + rec_words(rs) * m;
@@ -618,7 +635,7 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 T = T * (1.0f / sp.rr_prob);
             }
             S.T[p] = f4(T);
-            const uint32_t slot = atomicAdd(&S.cnt[nxt][0], 1u);
+            const uint32_t slot = wave_slot(&S.cnt[nxt][0], 1u);
             store_ray(S.rays[nxt], slot, phit, nd, kEpsRay, __builtin_inff());
             S.act[nxt][slot] = p;
             if (BATCH) S.rsid[nxt][slot] = sid;

@@ -99,6 +99,23 @@ MCRT_HD float qplane(float origin, uint32_t word, int shift, float scale)
 }
 
 // Slab test of child `shift / 8` of a QNode against its decoded
+// (outward-rounded) box, given the plane words already ordered near / far
+// per axis by the ray's signs: the same operations, in the same order, as
+// slab() on the decoded box.
+MCRT_HD bool qslab_nf(const RayPrep& r, uint32_t child, uint32_t nx, uint32_t fx, uint32_t ny, uint32_t fy,
+                      uint32_t nz, uint32_t fz, float ox, float oy, float oz, float sx, float sy, float sz, int shift,
+                      float R1, float& lo)
+{
+    if (child == MCRT_CHILD_EMPTY) return false;
+    const float t0x = (qplane(ox, nx, shift, sx) - r.ox) * r.ix, t1x = (qplane(ox, fx, shift, sx) - r.ox) * r.ix;
+    const float t0y = (qplane(oy, ny, shift, sy) - r.oy) * r.iy, t1y = (qplane(oy, fy, shift, sy) - r.oy) * r.iy;
+    const float t0z = (qplane(oz, nz, shift, sz) - r.oz) * r.iz, t1z = (qplane(oz, fz, shift, sz) - r.oz) * r.iz;
+    lo = fmaxf(fmaxf(t0x, t0y), t0z);
+    const float hi = fminf(fminf(t1x, t1y), t1z);
+    return lo <= hi && lo <= R1 && hi >= r.R0;
+}
+
+// Slab test of child `shift / 8` of a QNode against its decoded
 // (outward-rounded) box.
 MCRT_HD bool qslab(const RayPrep& r, uint32_t child, const uint4& qa, const uint4& qb, float ox, float oy, float oz,
                    float sx, float sy, float sz, int shift, float R1, float& lo)
@@ -135,9 +152,10 @@ struct TraceStack
     uint32_t* node;  // LDS, [k * stride + lane]
     float* lo;
     int stride, lane, depth;
-    uint64_t* spill;  // global, [spill_lane * spill_depth + (k - depth)]
+    uint64_t* spill;  // global, [(k - depth) * spill_stride + spill_lane]: the lanes of a
+                      // wavefront spilling the same entry write one contiguous run
     uint32_t spill_lane;
-    int spill_depth;
+    uint32_t spill_stride;  // lanes the spill area covers
 
     MCRT_HD void push(int sp, uint32_t n, float l)
     {
@@ -148,7 +166,7 @@ struct TraceStack
         }
         else
         {
-            spill[(size_t)spill_lane * spill_depth + (sp - depth)] =
+            spill[(size_t)(sp - depth) * spill_stride + spill_lane] =
                 ((uint64_t)bits_of(l) << 32) | (uint64_t)n;
         }
     }
@@ -161,7 +179,7 @@ struct TraceStack
         }
         else
         {
-            uint64_t x = spill[(size_t)spill_lane * spill_depth + (sp - depth)];
+            uint64_t x = spill[(size_t)(sp - depth) * spill_stride + spill_lane];
             n = (uint32_t)x;
             l = float_of((uint32_t)(x >> 32));
         }
@@ -232,10 +250,15 @@ struct Tracer
             bool p0, p1, p2, p3;
             if (!r.exact)
             {
-                p0 = qslab(r, ch.x, qa, qb, ox, oy, oz, sx, sy, sz, 0, R1, l0);
-                p1 = qslab(r, ch.y, qa, qb, ox, oy, oz, sx, sy, sz, 8, R1, l1);
-                p2 = qslab(r, ch.z, qa, qb, ox, oy, oz, sx, sy, sz, 16, R1, l2);
-                p3 = qslab(r, ch.w, qa, qb, ox, oy, oz, sx, sy, sz, 24, R1, l3);
+                // near and far planes by the ray's signs (the swap of
+                // aabb.cpp's slab test), chosen once for all four children
+                const uint32_t nx = r.sx ? qa.y : qa.x, fx = r.sx ? qa.x : qa.y;
+                const uint32_t ny = r.sy ? qa.w : qa.z, fy = r.sy ? qa.z : qa.w;
+                const uint32_t nz = r.sz ? qb.y : qb.x, fz = r.sz ? qb.x : qb.y;
+                p0 = qslab_nf(r, ch.x, nx, fx, ny, fy, nz, fz, ox, oy, oz, sx, sy, sz, 0, R1, l0);
+                p1 = qslab_nf(r, ch.y, nx, fx, ny, fy, nz, fz, ox, oy, oz, sx, sy, sz, 8, R1, l1);
+                p2 = qslab_nf(r, ch.z, nx, fx, ny, fy, nz, fz, ox, oy, oz, sx, sy, sz, 16, R1, l2);
+                p3 = qslab_nf(r, ch.w, nx, fx, ny, fy, nz, fz, ox, oy, oz, sx, sy, sz, 24, R1, l3);
             }
             else
             {
@@ -247,24 +270,24 @@ struct Tracer
                 p3 = ch.w != MCRT_CHILD_EMPTY;
                 l0 = l1 = l2 = l3 = -__builtin_inff();
             }
-            const uint32_t ord = ((r.oct & 4u) ? qb.w : qb.z) >> ((r.oct & 3u) * 8u);
-            uint32_t next = MCRT_CHILD_EMPTY;
-            float next_lo = 0.0f;
-#pragma unroll
-            for (int k = 3; k >= 0; --k)
+            // Each slot's position in this octant's visit order; the passing
+            // children as a mask over positions.  The nearest is visited
+            // next; the others go on the stack farthest first, i.e. a child
+            // lands above as many entries as passing children follow it.
+            const uint32_t rk = ((r.oct & 4u) ? qb.w : qb.z) >> ((r.oct & 3u) * 8u);
+            const uint32_t r0 = rk & 3u, r1 = (rk >> 2) & 3u, r2 = (rk >> 4) & 3u, r3 = (rk >> 6) & 3u;
+            const uint32_t m = (p0 ? 1u << r0 : 0u) | (p1 ? 1u << r1 : 0u) | (p2 ? 1u << r2 : 0u) |
+                               (p3 ? 1u << r3 : 0u);
+            if (m != 0)
             {
-                const uint32_t s = (ord >> (2 * k)) & 3u;
-                const bool ps = s == 0 ? p0 : s == 1 ? p1 : s == 2 ? p2 : p3;
-                if (ps)
-                {
-                    if (next != MCRT_CHILD_EMPTY) { st.push(sp, next, next_lo); ++sp; }
-                    next = s == 0 ? ch.x : s == 1 ? ch.y : s == 2 ? ch.z : ch.w;
-                    next_lo = s == 0 ? l0 : s == 1 ? l1 : s == 2 ? l2 : l3;
-                }
-            }
-            if (next != MCRT_CHILD_EMPTY)
-            {
-                cur = next;
+                const uint32_t first = (uint32_t)__builtin_ctz(m);
+                const int c = __builtin_popcount(m);
+                cur = (p0 && r0 == first) ? ch.x : (p1 && r1 == first) ? ch.y : (p2 && r2 == first) ? ch.z : ch.w;
+                if (p0 && r0 != first) st.push(sp + __builtin_popcount(m >> (r0 + 1u)), ch.x, l0);
+                if (p1 && r1 != first) st.push(sp + __builtin_popcount(m >> (r1 + 1u)), ch.y, l1);
+                if (p2 && r2 != first) st.push(sp + __builtin_popcount(m >> (r2 + 1u)), ch.z, l2);
+                if (p3 && r3 != first) st.push(sp + __builtin_popcount(m >> (r3 + 1u)), ch.w, l3);
+                sp += c - 1;
                 return false;
             }
         }

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
     st.depth = D;
     st.spill = S.spill;
     st.spill_lane = blockIdx.x * kWave + threadIdx.x;
-    st.spill_depth = S.spill_depth;
+    st.spill_stride = (uint32_t)S.spill_lanes;  // >= the grid's lanes (trace_blocks)
     // 32-bit ray indices: a launch covers at most 2^31 rays (launch_trace
     // splits larger batches; path-tracer chunks are at most 5 * 2^28 rays),
     // and a slice counter ends below its slice + 2 * kClaim * (waves of the grid).

@@ -296,8 +296,8 @@ void quantize_nodes(WideBVH& out)
             q.q_hi_x = q.q_hi_y = q.q_hi_z = 0xffffffffu;
         }
         q.exps = e[0] | (e[1] << 8) | (e[2] << 16) | (w.nchild << 24);
-        q.order_lo = w.order_lo;
-        q.order_hi = w.order_hi;
+        q.rank_lo = order_to_rank(w.order_lo);
+        q.rank_hi = order_to_rank(w.order_hi);
     }
 }
 }  // namespace mcrt

@@ -65,7 +65,7 @@ extern "C" void emu_free(void* h) { delete (Emu*)h; }
 
 // Quantised nodes against the exact ones: out[0] = planes checked, out[1] =
 // decoded planes that do not contain the exact child box, out[2] = child
-// links or order tables that differ, out[3] = sum over checked planes of
+// links or rank tables (inverse order tables) that differ, out[3] = sum over checked planes of
 // (decoded - exact) outward growth relative to the node extent, x 1e6.
 extern "C" void emu_check_qnodes(void* h, long long* out)
 {
@@ -76,7 +76,13 @@ extern "C" void emu_check_qnodes(void* h, long long* out)
     {
         const mcrt::WNode& w = e->wb.nodes[i];
         const mcrt::QNode& q = e->wb.qnodes[i];
-        if (q.order_lo != w.order_lo || q.order_hi != w.order_hi) links++;
+        for (int o = 0; o < 8; o++)
+            for (int k = 0; k < 4; k++)
+            {
+                const uint32_t ord = o < 4 ? w.order_lo : w.order_hi, rk = o < 4 ? q.rank_lo : q.rank_hi;
+                const uint32_t s = (ord >> (8 * (o & 3) + 2 * k)) & 3u;
+                if (((rk >> (8 * (o & 3) + 2 * s)) & 3u) != (uint32_t)k) links++;
+            }
         const float* lo[3] = {w.lo_x, w.lo_y, w.lo_z};
         const float* hi[3] = {w.hi_x, w.hi_y, w.hi_z};
         const uint32_t qlo[3] = {q.q_lo_x, q.q_lo_y, q.q_lo_z};
@@ -134,7 +140,7 @@ extern "C" void emu_trace(void* h, const float* rays, long long n, int* hits, un
         st.depth = lds_depth;
         st.spill = spill.data();
         st.spill_lane = 0;
-        st.spill_depth = spill_depth;
+        st.spill_stride = 1;
         while (true)
         {
             long long b = next.fetch_add(1024);
