@@ -127,7 +127,7 @@ struct CacheHeader
     uint64_t tree_digest, payload_hash;
 };
 const char kCacheMagic[8] = {'M', 'C', 'R', 'T', 'B', 'V', 'H', '\0'};
-const uint32_t kCacheVersion = 1;
+const uint32_t kCacheVersion = 2;  // 2: rank tables, single-reference children
 
 // FNV-1a over 64-bit words (bytes beyond a multiple of 8 folded in last).
 uint64_t hash_bytes(const void* p, size_t n, uint64_t h = 0xcbf29ce484222325ull)

@@ -1,14 +1,17 @@
 // Binary SBVH -> 4-wide nodes.  A wide node is a treelet of up to three
 // binary branches whose frontier (up to four binary nodes) becomes its
 // children; the branch to open next is the one with the largest surface
-// area.  Dropping the treelet's inner boxes is exact: a child box lies
-// inside its parent's box (node box = union of its references' clipped
-// boxes, and every child reference's box lies inside a parent reference's
-// box), and the slab interval of a contained box is contained after
-// rounding, so a child passes only if its parent passes.  Leaf boxes are
-// likewise implied by the boxes of their references.  The visit order of
-// the binary traversal (near child first iff dir[axis] > 0,
-// splitbvh.cpp:73-134) is stored per ray octant.
+// area -- a branch, or a leaf whose references fit in the free slots, which
+// then become single-reference children (each with its own clipped box,
+// tested in the node instead of being fetched to be tested).  Dropping the
+// treelet's inner boxes is exact: a child box lies inside its parent's box
+// (node box = union of its references' clipped boxes, and every child
+// reference's box lies inside a parent reference's box), and the slab
+// interval of a contained box is contained after rounding, so a child
+// passes only if its parent passes.  Leaf boxes are likewise implied by
+// the boxes of their references.  The visit order of the binary traversal
+// (near child first iff dir[axis] > 0, splitbvh.cpp:73-134; a leaf's
+// references in slot order) is stored per ray octant.
 #include "wide.h"
 
 #include <algorithm>
@@ -22,7 +25,7 @@ namespace
 {
 struct TL
 {
-    int axis;
+    int axis;  // split axis; -1: a leaf's references, l before r in every octant
     int l, r;  // >= 0: treelet node; < 0: slot -(x+1)
 };
 
@@ -35,8 +38,8 @@ struct Builder
     void dfs_slots(const std::vector<TL>& tl, int k, std::vector<int>& seq, const int pos[3]) const
     {
         const TL& t = tl[k];
-        int first = pos[t.axis] ? t.l : t.r;
-        int second = pos[t.axis] ? t.r : t.l;
+        int first = (t.axis < 0 || pos[t.axis]) ? t.l : t.r;
+        int second = (t.axis < 0 || pos[t.axis]) ? t.r : t.l;
         for (int x : {first, second})
         {
             if (x >= 0) dfs_slots(tl, x, seq, pos);
@@ -44,10 +47,11 @@ struct Builder
         }
     }
 
-    uint32_t leaf_link(const BNode& b)
+    uint32_t leaf_link(const BNode& b, int one = -1)
     {
         uint32_t first = (uint32_t)out.refs.size();
-        for (int k = 0; k < b.ref_count; k++)
+        const int k0 = one >= 0 ? one : 0, k1 = one >= 0 ? one + 1 : b.ref_count;
+        for (int k = k0; k < k1; k++)
         {
             const SRef& s = bt.refs[b.ref_begin + k];
             const Tri& t = tris[s.tri];
@@ -63,7 +67,7 @@ struct Builder
             w.prim = s.tri;
             out.refs.push_back(w);
         }
-        return MCRT_CHILD_LEAF | ((uint32_t)(b.ref_count - 1) << 29) | first;
+        return MCRT_CHILD_LEAF | ((uint32_t)(k1 - k0 - 1) << 29) | first;
     }
 
     // Returns the wide index and writes the subtree's stack need and depth.
@@ -71,31 +75,64 @@ struct Builder
                  int& depth)
     {
         std::vector<int32_t> slot = frontier_in;
+        std::vector<int> sref(slot.size(), -1);  // >= 0: the slot is that reference of its leaf
         std::vector<TL> tl = tl_in;
-        // Open the largest-area branch until four children.
+        auto hook = [&](int from, int tk) {
+            for (size_t k = 0; k + 1 < tl.size(); k++)
+            {
+                if (tl[k].l == -(from + 1)) tl[k].l = tk;
+                else if (tl[k].r == -(from + 1)) tl[k].r = tk;
+            }
+        };
+        // Open the largest-area branch, or expand the largest-area leaf whose
+        // references fit, until four children.
         while (slot.size() < 4)
         {
-            int best = -1;
-            float best_a = -1.0f;
+            int best = -1, best_leaf = -1;
+            float best_a = -1.0f, best_la = -1.0f;
             for (size_t i = 0; i < slot.size(); i++)
             {
+                if (sref[i] >= 0) continue;
                 const BNode& b = bt.nodes[slot[i]];
-                if (b.leaf) continue;
                 float a = b.box.half_area();
+                if (b.leaf)
+                {
+                    if (b.ref_count >= 2 && slot.size() + b.ref_count - 1 <= 4 && a > best_la)
+                    {
+                        best_la = a;
+                        best_leaf = (int)i;
+                    }
+                    continue;
+                }
                 if (a > best_a) { best_a = a; best = (int)i; }
+            }
+            if (best_leaf >= 0 && (best < 0 || best_la > best_a))
+            {
+                // a leaf's references become children, in slot order
+                const BNode& b = bt.nodes[slot[best_leaf]];
+                const int base = (int)slot.size();
+                sref[best_leaf] = 0;
+                int prev_slot = best_leaf;
+                for (int k = 1; k < b.ref_count; k++)
+                {
+                    slot.push_back(slot[best_leaf]);
+                    sref.push_back(k);
+                    const int tk = (int)tl.size();
+                    tl.push_back(TL{-1, -(prev_slot + 1), -(base + k - 1 + 1)});
+                    hook(prev_slot, tk);
+                    prev_slot = base + k - 1;
+                }
+                continue;
             }
             if (best < 0) break;
             const BNode& b = bt.nodes[slot[best]];
             int ns = (int)slot.size();
             slot[best] = b.left;
             slot.push_back(b.right);
+            sref.push_back(-1);
             int tk = (int)tl.size();
             tl.push_back(TL{b.axis, -(best + 1), -(ns + 1)});
-            for (size_t k = 0; k + 1 < tl.size(); k++)
-            {
-                if (tl[k].l == -(best + 1)) tl[k].l = tk;
-                else if (tl[k].r == -(best + 1)) tl[k].r = tk;
-            }
+            hook(best, tk);
         }
         // Renumber slots in left-first treelet order.
         std::vector<int> canon;
@@ -105,7 +142,13 @@ struct Builder
         }
         std::vector<int> remap(slot.size());
         std::vector<int32_t> cslot(slot.size());
-        for (size_t i = 0; i < canon.size(); i++) { remap[canon[i]] = (int)i; cslot[i] = slot[canon[i]]; }
+        std::vector<int> csref(slot.size());
+        for (size_t i = 0; i < canon.size(); i++)
+        {
+            remap[canon[i]] = (int)i;
+            cslot[i] = slot[canon[i]];
+            csref[i] = sref[canon[i]];
+        }
         for (TL& t : tl)
         {
             if (t.l < 0) t.l = -(remap[-t.l - 1] + 1);
@@ -138,13 +181,15 @@ struct Builder
             if (s >= (int)cslot.size()) continue;
             const BNode& b = bt.nodes[cslot[s]];
             if (b.leaf && b.ref_count == 0) continue;  // cannot produce a hit
-            w.lo_x[s] = b.box.lo[0]; w.hi_x[s] = b.box.hi[0];
-            w.lo_y[s] = b.box.lo[1]; w.hi_y[s] = b.box.hi[1];
-            w.lo_z[s] = b.box.lo[2]; w.hi_z[s] = b.box.hi[2];
+            // a single reference's child box is its clipped box
+            const Box& cb = csref[s] >= 0 ? bt.refs[b.ref_begin + csref[s]].box : b.box;
+            w.lo_x[s] = cb.lo[0]; w.hi_x[s] = cb.hi[0];
+            w.lo_y[s] = cb.lo[1]; w.hi_y[s] = cb.hi[1];
+            w.lo_z[s] = cb.lo[2]; w.hi_z[s] = cb.hi[2];
             nchild++;
             if (b.leaf)
             {
-                w.child[s] = leaf_link(b);
+                w.child[s] = leaf_link(b, csref[s]);
                 continue;
             }
             int cn = 0, cd = 0;

@@ -27,7 +27,7 @@ def test_traversal_tuning_bit_exact(ctx, depth, waves):
     import mcrt_amd
     print(f"[tuning] depth={depth} waves={waves}", flush=True)
     tris, s, st = _debug_scene(ctx)
-    assert st["stack_bound"] == 23, "fixture spills at LDS depths 8, 12 and 16"
+    assert st["stack_bound"] > 16, "fixture spills at LDS depths 8, 12 and 16"
     s.set_tuning(trace_waves_per_cu=waves, trace_lds_depth=depth)
     assert s.tuning()["trace_lds_depth"] == depth and s.tuning()["trace_waves_per_cu"] == waves
     rays = _libs.random_rays(tris, 1 << 15, seed=11)
