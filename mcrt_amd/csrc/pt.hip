@@ -376,14 +376,15 @@ __device__ __forceinline__ V3f area_terms(const DevScene& D, const int4 h1, cons
 
 // Next-event contributions of one diffuse vertex: the environment terms
 // (demo_pt.cpp:564-589), each counted when its shadow ray escapes, then the
-// area-light terms, in the reference's order of accumulation.
-template <bool ADJ, bool BATCH>
+// area-light terms, in the reference's order of accumulation.  ENV: the
+// records hold the environment terms (rs = 4).
+template <bool ADJ, bool BATCH, bool ENV>
 __global__ void k_resolve(PTArgs a, PTState S, int cur)
 {
     __shared__ float acc[ADJ ? 2 * kHotFloats : 1];
     if (ADJ) hot_clear(acc);
     const int64_t P = a.P;
-    const int rs = a.rs;
+    constexpr int rs = ENV ? 4 : 2;
     const uint32_t nrec = S.cnt[cur][1] / (uint32_t)rs;
     const float4* R = S.rec[cur];
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nrec; base += (int64_t)gridDim.x * blockDim.x)
@@ -401,7 +402,7 @@ __global__ void k_resolve(PTArgs a, PTState S, int cur)
             const DevScene& D = BATCH ? a.tab[S.sid[path]].D : a.one.D;
             const int64_t h0 = P + rs * m;
             V3f local = v3f(0.0f, 0.0f, 0.0f);
-            if (rs == 4)
+            if (ENV)
             {
                 const V3f z = v3f(0.0f, 0.0f, 0.0f);
                 const V3f e1 = S.hits[h0 + 2].x < 0 ? f3(Rm[4]) : z;
@@ -474,8 +475,11 @@ __device__ __forceinline__ void env_terms(const DevScene& D, uint64_t key, V3f n
     }
 }
 
-// One vertex of every live path (demo_pt.cpp:523-651).
-template <bool ADJ, bool BATCH>
+// One vertex of every live path (demo_pt.cpp:523-651).  ENV: a scene of the
+// launch has an environment map (escape radiance, rs = 4 next-event rays);
+// without one the environment code is compiled out, which keeps the
+// kernel's registers, and so its occupancy, those of area lights alone.
+template <bool ADJ, bool BATCH, bool ENV>
 __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
 {
     __shared__ float acc[ADJ ? 2 * kHotFloats : 1];
@@ -499,7 +503,7 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
             if (h.x < 0)
             {
                 // escape: the passive environment term (demo_pt.cpp:528-540)
-                if (D.env_w > 0 && (!sp.nee || k == 0 || !S.last_diff[p]))
+                if (ENV && D.env_w > 0 && (!sp.nee || k == 0 || !S.last_diff[p]))
                 {
                     const V3f Lv = f3(S.L[p]) + f3(S.T[p]) * env_rad(D, env_pixel(D, f3(rays[2 * j + 1])));
                     S.L[p] = f4(Lv);
@@ -552,9 +556,9 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 const uint64_t sample = (uint64_t)(sp.sample_offset + p % sp.spp);
                 key = rng_key(sp.seed, (uint64_t)pixel, sample, (uint32_t)k);
             }
-            if (sp.nee && mt.type == MCRT_DIFF && (D.nlights > 0 || D.env_w > 0))
+            if (sp.nee && mt.type == MCRT_DIFF && (D.nlights > 0 || (ENV && D.env_w > 0)))
             {
-                const int rs = a.rs;
+                constexpr int rs = ENV ? 4 : 2;
                 const uint32_t slot = wave_slot(&S.cnt[nxt][1], (uint32_t)rs);
                 const int64_t m = slot / (uint32_t)rs;
                 const int64_t r0i = a.P + rs * m;
@@ -601,7 +605,7 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 // rays a vertex does not need cost one root visit
                 store_ray(nr, r0i, phit, wb, kEpsRay, __builtin_inff());
                 store_ray(nr, r0i + 1, phit, li_rec >= 0 ? wo : zero, kEpsRay, __builtin_inff());
-                if (rs == 4)
+                if (ENV)
                 {
                     V3f e1 = zero, e2 = zero, d1 = zero, d2 = zero;
                     if (D.env_w > 0) env_terms(D, key, ns, col, e1, e2, d1, d2);
@@ -747,11 +751,17 @@ void bounce_loop(const DevScene& Dt, PTState& S, const PTArgs& a, int max_depth,
         launch_k_trace<false, false, true, BATCH>(Dt, trace_blocks(Dt, (1 + a.rs) * a.P), st, S.rays[cur], (int64_t)0,
                                                   S.cnt[cur], a.P, S.hits, (uint8_t*)nullptr,
                                                   (unsigned long long*)nullptr, S.cnt[cur] + kCntWq, tb);
-        hipLaunchKernelGGL((k_resolve<ADJ, BATCH>), dim3(g), dim3(kTPB), 0, st, a, S, cur);
+        if (a.rs == 4)
+            hipLaunchKernelGGL((k_resolve<ADJ, BATCH, true>), dim3(g), dim3(kTPB), 0, st, a, S, cur);
+        else
+            hipLaunchKernelGGL((k_resolve<ADJ, BATCH, false>), dim3(g), dim3(kTPB), 0, st, a, S, cur);
         if (k <= max_depth)
         {
             hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[nxt], 0u, 0u);
-            hipLaunchKernelGGL((k_shade<ADJ, BATCH>), dim3(g), dim3(kTPB), 0, st, a, S, cur, k);
+            if (a.rs == 4)
+                hipLaunchKernelGGL((k_shade<ADJ, BATCH, true>), dim3(g), dim3(kTPB), 0, st, a, S, cur, k);
+            else
+                hipLaunchKernelGGL((k_shade<ADJ, BATCH, false>), dim3(g), dim3(kTPB), 0, st, a, S, cur, k);
         }
         cur = nxt;
     }
