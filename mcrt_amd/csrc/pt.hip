@@ -725,21 +725,43 @@ __global__ void k_accumulate(PTState S, PTArgs a, float* film, int64_t npix)
     }
 }
 
-unsigned grid_for(int64_t n, int num_cus)
+// Workgroups of kTPB threads of kernel K resident at once on a CU (queried
+// once per kernel).
+template <auto K>
+int resident_per_cu()
+{
+    static const int n = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(K), kTPB, 0) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            b = 0;
+        }
+        return b < 1 ? 1 : b;
+    }();
+    return n;
+}
+
+// Grid of a grid-strided kernel over n items: at most the workgroups that
+// are resident at once, so all of them start together and none is left
+// for a second round (a grid a little larger than what fits doubles the
+// time of its last workgroups).
+template <auto K>
+dim3 grid_of(int64_t n, int num_cus)
 {
     int64_t b = (n + kTPB - 1) / kTPB;
-    int64_t cap = (int64_t)num_cus * 8;
+    const int64_t cap = (int64_t)num_cus * resident_per_cu<K>();
     if (b > cap) b = cap;
     if (b < 1) b = 1;
-    return (unsigned)b;
+    return dim3((unsigned)b);
 }
 
 // The bounce loop of one chunk (generate .. last resolve).  Dt carries the
 // traversal settings (LDS depth, spill stacks) of the launch.
 template <bool ADJ, bool BATCH>
-void bounce_loop(const DevScene& Dt, PTState& S, const PTArgs& a, int max_depth, unsigned g, hipStream_t st)
+void bounce_loop(const DevScene& Dt, PTState& S, const PTArgs& a, int max_depth, int cus, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_generate<BATCH>, dim3(g), dim3(kTPB), 0, st, S, a);
+    hipLaunchKernelGGL(k_generate<BATCH>, (grid_of<k_generate<BATCH>>(a.P, cus)), dim3(kTPB), 0, st, S, a);
     hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[0], (uint32_t)a.P, 0u);
     int cur = 0;
     for (int k = 0; k <= max_depth + 1; k++)
@@ -752,16 +774,20 @@ void bounce_loop(const DevScene& Dt, PTState& S, const PTArgs& a, int max_depth,
                                                   S.cnt[cur], a.P, S.hits, (uint8_t*)nullptr,
                                                   (unsigned long long*)nullptr, S.cnt[cur] + kCntWq, tb);
         if (a.rs == 4)
-            hipLaunchKernelGGL((k_resolve<ADJ, BATCH, true>), dim3(g), dim3(kTPB), 0, st, a, S, cur);
+            hipLaunchKernelGGL((k_resolve<ADJ, BATCH, true>), (grid_of<k_resolve<ADJ, BATCH, true>>(a.P, cus)),
+                               dim3(kTPB), 0, st, a, S, cur);
         else
-            hipLaunchKernelGGL((k_resolve<ADJ, BATCH, false>), dim3(g), dim3(kTPB), 0, st, a, S, cur);
+            hipLaunchKernelGGL((k_resolve<ADJ, BATCH, false>), (grid_of<k_resolve<ADJ, BATCH, false>>(a.P, cus)),
+                               dim3(kTPB), 0, st, a, S, cur);
         if (k <= max_depth)
         {
             hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, S.cnt[nxt], 0u, 0u);
             if (a.rs == 4)
-                hipLaunchKernelGGL((k_shade<ADJ, BATCH, true>), dim3(g), dim3(kTPB), 0, st, a, S, cur, k);
+                hipLaunchKernelGGL((k_shade<ADJ, BATCH, true>), (grid_of<k_shade<ADJ, BATCH, true>>(a.P, cus)),
+                                   dim3(kTPB), 0, st, a, S, cur, k);
             else
-                hipLaunchKernelGGL((k_shade<ADJ, BATCH, false>), dim3(g), dim3(kTPB), 0, st, a, S, cur, k);
+                hipLaunchKernelGGL((k_shade<ADJ, BATCH, false>), (grid_of<k_shade<ADJ, BATCH, false>>(a.P, cus)),
+                                   dim3(kTPB), 0, st, a, S, cur, k);
         }
         cur = nxt;
     }
@@ -867,8 +893,8 @@ int render_scene(mcrt_scene* s, StreamRes* r, const mcrt_render_params* prm, flo
     if (adj)
     {
         const int64_t ng = 3 * s->dev.ntri;
-        hipLaunchKernelGGL(k_fill, dim3(grid_for(ng, cus)), dim3(kTPB), 0, st, adj->d_albedo, ng, 0.0f);
-        hipLaunchKernelGGL(k_fill, dim3(grid_for(ng, cus)), dim3(kTPB), 0, st, adj->d_emission, ng, 0.0f);
+        hipLaunchKernelGGL(k_fill, (grid_of<k_fill>(ng, cus)), dim3(kTPB), 0, st, adj->d_albedo, ng, 0.0f);
+        hipLaunchKernelGGL(k_fill, (grid_of<k_fill>(ng, cus)), dim3(kTPB), 0, st, adj->d_emission, ng, 0.0f);
     }
     PTArgs a;
     a.nseg = 0;
@@ -882,17 +908,18 @@ int render_scene(mcrt_scene* s, StreamRes* r, const mcrt_render_params* prm, flo
         const int64_t np = std::min(pix_per_chunk, npix - pix0);
         a.P = np * spp;
         a.pix0 = pix0;
-        const unsigned g = grid_for(a.P, cus);
-        bounce_loop<false, false>(D, S, a, prm->max_depth, g, st);
+        bounce_loop<false, false>(D, S, a, prm->max_depth, cus, st);
         if (adj)
         {
-            hipLaunchKernelGGL(k_delta, dim3(g), dim3(kTPB), 0, st, S, a, adj->dfilm, a.one.scale);
-            bounce_loop<true, false>(D, S, a, prm->max_depth, g, st);
+            hipLaunchKernelGGL(k_delta, (grid_of<k_delta>(a.P, cus)), dim3(kTPB), 0, st, S, a, adj->dfilm,
+                               a.one.scale);
+            bounce_loop<true, false>(D, S, a, prm->max_depth, cus, st);
         }
         if (film)
         {
-            hipLaunchKernelGGL(k_finish, dim3(g), dim3(kTPB), 0, st, S, a.P);
-            hipLaunchKernelGGL(k_accumulate<false>, dim3(grid_for(np, cus)), dim3(kTPB), 0, st, S, a, film, np);
+            hipLaunchKernelGGL(k_finish, (grid_of<k_finish>(a.P, cus)), dim3(kTPB), 0, st, S, a.P);
+            hipLaunchKernelGGL(k_accumulate<false>, (grid_of<k_accumulate<false>>(np, cus)), dim3(kTPB), 0, st, S, a,
+                               film, np);
         }
         HIPCHK(hipGetLastError());
     }
@@ -997,10 +1024,9 @@ int batch_enqueue(mcrt_batch* b, hipStream_t st)
         a.P = c.P;
         a.nseg = (int32_t)c.nseg;
         a.seg = b->d_seg.as<Segment>() + c.seg0;
-        const unsigned g = grid_for(a.P, cus);
-        bounce_loop<false, true>(b->trace_view, S, a, b->max_depth, g, st);
-        hipLaunchKernelGGL(k_finish, dim3(g), dim3(kTPB), 0, st, S, a.P);
-        hipLaunchKernelGGL(k_accumulate<true>, dim3(grid_for(c.npix, cus)), dim3(kTPB), 0, st, S, a,
+        bounce_loop<false, true>(b->trace_view, S, a, b->max_depth, cus, st);
+        hipLaunchKernelGGL(k_finish, (grid_of<k_finish>(a.P, cus)), dim3(kTPB), 0, st, S, a.P);
+        hipLaunchKernelGGL(k_accumulate<true>, (grid_of<k_accumulate<true>>(c.npix, cus)), dim3(kTPB), 0, st, S, a,
                            (float*)nullptr, c.npix);
         HIPCHK(hipGetLastError());
     }
