@@ -1,9 +1,9 @@
 // Binary SBVH -> 4-wide nodes.  A wide node is a treelet of up to three
 // binary branches whose frontier (up to four binary nodes) becomes its
 // children; the branch to open next is the one with the largest surface
-// area -- a branch, or a leaf whose references fit in the free slots, which
-// then become single-reference children (each with its own clipped box,
-// tested in the node instead of being fetched to be tested).  Dropping the
+// area -- a branch, or a leaf, whose references then become children (each
+// with its own clipped box, tested in the node instead of being fetched to
+// be tested; when they do not all fit, the last child holds the rest).  Dropping the
 // treelet's inner boxes is exact: a child box lies inside its parent's box
 // (node box = union of its references' clipped boxes, and every child
 // reference's box lies inside a parent reference's box), and the slab
@@ -47,10 +47,11 @@ struct Builder
         }
     }
 
-    uint32_t leaf_link(const BNode& b, int one = -1)
+    // Leaf link to references [k0, k0 + cnt) of leaf b (cnt = 0: all of them).
+    uint32_t leaf_link(const BNode& b, int k0 = 0, int cnt = 0)
     {
         uint32_t first = (uint32_t)out.refs.size();
-        const int k0 = one >= 0 ? one : 0, k1 = one >= 0 ? one + 1 : b.ref_count;
+        const int k1 = cnt > 0 ? k0 + cnt : b.ref_count;
         for (int k = k0; k < k1; k++)
         {
             const SRef& s = bt.refs[b.ref_begin + k];
@@ -75,7 +76,8 @@ struct Builder
                  int& depth)
     {
         std::vector<int32_t> slot = frontier_in;
-        std::vector<int> sref(slot.size(), -1);  // >= 0: the slot is that reference of its leaf
+        // sref >= 0: the slot holds references [sref, sref + scnt) of its leaf
+        std::vector<int> sref(slot.size(), -1), scnt(slot.size(), 0);
         std::vector<TL> tl = tl_in;
         auto hook = [&](int from, int tk) {
             for (size_t k = 0; k + 1 < tl.size(); k++)
@@ -84,8 +86,8 @@ struct Builder
                 else if (tl[k].r == -(from + 1)) tl[k].r = tk;
             }
         };
-        // Open the largest-area branch, or expand the largest-area leaf whose
-        // references fit, until four children.
+        // Open the largest-area branch, or expand the largest-area leaf into
+        // its references, until four children.
         while (slot.size() < 4)
         {
             int best = -1, best_leaf = -1;
@@ -97,7 +99,7 @@ struct Builder
                 float a = b.box.half_area();
                 if (b.leaf)
                 {
-                    if (b.ref_count >= 2 && slot.size() + b.ref_count - 1 <= 4 && a > best_la)
+                    if (b.ref_count >= 2 && a > best_la)
                     {
                         best_la = a;
                         best_leaf = (int)i;
@@ -108,15 +110,19 @@ struct Builder
             }
             if (best_leaf >= 0 && (best < 0 || best_la > best_a))
             {
-                // a leaf's references become children, in slot order
+                // a leaf's references become children, in slot order; when
+                // they do not all fit, the last child holds the rest
                 const BNode& b = bt.nodes[slot[best_leaf]];
                 const int base = (int)slot.size();
+                const int items = std::min(b.ref_count, 4 - base + 1);
                 sref[best_leaf] = 0;
+                scnt[best_leaf] = 1;
                 int prev_slot = best_leaf;
-                for (int k = 1; k < b.ref_count; k++)
+                for (int k = 1; k < items; k++)
                 {
                     slot.push_back(slot[best_leaf]);
                     sref.push_back(k);
+                    scnt.push_back(k + 1 < items ? 1 : b.ref_count - k);
                     const int tk = (int)tl.size();
                     tl.push_back(TL{-1, -(prev_slot + 1), -(base + k - 1 + 1)});
                     hook(prev_slot, tk);
@@ -130,6 +136,7 @@ struct Builder
             slot[best] = b.left;
             slot.push_back(b.right);
             sref.push_back(-1);
+            scnt.push_back(0);
             int tk = (int)tl.size();
             tl.push_back(TL{b.axis, -(best + 1), -(ns + 1)});
             hook(best, tk);
@@ -142,12 +149,13 @@ struct Builder
         }
         std::vector<int> remap(slot.size());
         std::vector<int32_t> cslot(slot.size());
-        std::vector<int> csref(slot.size());
+        std::vector<int> csref(slot.size()), cscnt(slot.size());
         for (size_t i = 0; i < canon.size(); i++)
         {
             remap[canon[i]] = (int)i;
             cslot[i] = slot[canon[i]];
             csref[i] = sref[canon[i]];
+            cscnt[i] = scnt[canon[i]];
         }
         for (TL& t : tl)
         {
@@ -181,15 +189,20 @@ struct Builder
             if (s >= (int)cslot.size()) continue;
             const BNode& b = bt.nodes[cslot[s]];
             if (b.leaf && b.ref_count == 0) continue;  // cannot produce a hit
-            // a single reference's child box is its clipped box
-            const Box& cb = csref[s] >= 0 ? bt.refs[b.ref_begin + csref[s]].box : b.box;
+            // references as a child: the union of their clipped boxes
+            Box cb = b.box;
+            if (csref[s] >= 0)
+            {
+                cb = bt.refs[b.ref_begin + csref[s]].box;
+                for (int k = 1; k < cscnt[s]; k++) cb.grow(bt.refs[b.ref_begin + csref[s] + k].box);
+            }
             w.lo_x[s] = cb.lo[0]; w.hi_x[s] = cb.hi[0];
             w.lo_y[s] = cb.lo[1]; w.hi_y[s] = cb.hi[1];
             w.lo_z[s] = cb.lo[2]; w.hi_z[s] = cb.hi[2];
             nchild++;
             if (b.leaf)
             {
-                w.child[s] = leaf_link(b, csref[s]);
+                w.child[s] = csref[s] >= 0 ? leaf_link(b, csref[s], cscnt[s]) : leaf_link(b);
                 continue;
             }
             int cn = 0, cd = 0;

@@ -124,7 +124,7 @@ def test_visit_counts_vs_oracle(emu, which):
     _, cnt = Emu(tris).trace(rays)
     _, _, oc = _libs.OracleBVH(tris).intersect(rays, counters=True)
     assert cnt[2] == oc[2]                  # same Moller-Trumbore tests
-    assert 0.7 * oc[1] <= cnt[1] <= 1.1 * oc[1]
+    assert 0.5 * oc[1] <= cnt[1] <= 1.1 * oc[1]
     assert cnt[0] < oc[0]                   # fewer node fetches than binary box tests
 
 
