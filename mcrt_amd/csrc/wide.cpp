@@ -1,7 +1,8 @@
 // Binary SBVH -> 4-wide nodes.  A wide node is a treelet of up to three
 // binary branches whose frontier (up to four binary nodes) becomes its
-// children; the branch to open next is the one with the largest surface
-// area -- a branch, or a leaf, whose references then become children (each
+// children.  The frontier grows by opening the largest-area branch, or by
+// expanding the largest-area leaf into its references -- preferred unless
+// a branch is over four times larger -- which then become children (each
 // with its own clipped box, tested in the node instead of being fetched to
 // be tested; when they do not all fit, the last child holds the rest).  Dropping the
 // treelet's inner boxes is exact: a child box lies inside its parent's box
@@ -108,7 +109,10 @@ struct Builder
                 }
                 if (a > best_a) { best_a = a; best = (int)i; }
             }
-            if (best_leaf >= 0 && (best < 0 || best_la > best_a))
+            // leaves first unless a branch is over four times larger: a
+            // reference tested in the node saves a 64 B fetch and a step
+            // whenever its box misses (measured 0.25..0.1 best, 1 slower)
+            if (best_leaf >= 0 && (best < 0 || best_la > 0.25f * best_a))
             {
                 // a leaf's references become children, in slot order; when
                 // they do not all fit, the last child holds the rest
