@@ -134,3 +134,23 @@ def test_errors(ctx):
     with pytest.raises(mcrt_amd.McrtError) as e:
         s.load_model("/nonexistent/file.obj", s.add_material(mcrt_amd.DIFF, (1, 1, 1)))
     assert e.value.code == 3
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 8, 9, 127, 128, 129, 1023, 1025, 8 * 128 + 3, 40961])
+def test_batch_sizes_around_work_claims(ctx, n):
+    """Ray counts around the traversal's work split (eight slices, claims of
+    128 rays; below 64 rays the grid is one wave, which walks all eight
+    slices): every ray traced once, results equal the oracle's, and the
+    visit counters see exactly n rays."""
+    import mcrt_amd
+    tris = _libs.debug_mesh()
+    rays = _libs.random_rays(tris, n, seed=n)
+    s, _ = _scene(ctx, tris)
+    prim, tuv = mcrt_amd.split_hits(s.intersect(rays))
+    oprim, otuv = _libs.OracleBVH(tris).intersect(rays)
+    assert np.array_equal(prim, oprim)
+    assert np.array_equal(tuv.view(np.uint32), otuv.view(np.uint32))
+    assert s.query_counters(rays)["rays"] == n
+    for waves in (1, 3):
+        s.set_tuning(trace_waves_per_cu=waves)
+        assert np.array_equal(mcrt_amd.split_hits(s.intersect(rays))[0], oprim)
