@@ -106,13 +106,14 @@ MCRT_HD bool qslab_nf(const RayPrep& r, uint32_t child, uint32_t nx, uint32_t fx
                       uint32_t nz, uint32_t fz, float ox, float oy, float oz, float sx, float sy, float sz, int shift,
                       float R1, float& lo)
 {
-    if (child == MCRT_CHILD_EMPTY) return false;
+    // computed for every slot (an empty slot's planes decode to finite
+    // values), so the four tests need no branches
     const float t0x = (qplane(ox, nx, shift, sx) - r.ox) * r.ix, t1x = (qplane(ox, fx, shift, sx) - r.ox) * r.ix;
     const float t0y = (qplane(oy, ny, shift, sy) - r.oy) * r.iy, t1y = (qplane(oy, fy, shift, sy) - r.oy) * r.iy;
     const float t0z = (qplane(oz, nz, shift, sz) - r.oz) * r.iz, t1z = (qplane(oz, fz, shift, sz) - r.oz) * r.iz;
     lo = fmaxf(fmaxf(t0x, t0y), t0z);
     const float hi = fminf(fminf(t1x, t1y), t1z);
-    return lo <= hi && lo <= R1 && hi >= r.R0;
+    return (lo <= hi) & (lo <= R1) & (hi >= r.R0) & (child != MCRT_CHILD_EMPTY);
 }
 
 // Slab test of child `shift / 8` of a QNode against its decoded
