@@ -150,7 +150,7 @@ MCRT_HD bool mt(const RayPrep& r, float v0x, float v0y, float v0z, float e1x, fl
 
 struct TraceStack
 {
-    uint32_t* node;  // LDS, [k * stride + lane]
+    uint32_t* node;  // LDS, [k * stride + lane], depth + 1 rows: row `depth` discards writes
     float* lo;
     int stride, lane, depth;
     uint64_t* spill;  // global, [(k - depth) * spill_stride + spill_lane]: the lanes of a
@@ -158,6 +158,12 @@ struct TraceStack
     uint32_t spill_lane;
     uint32_t spill_stride;  // lanes the spill area covers
 
+    // Unconditional LDS write to row k <= depth (row depth: discarded).
+    MCRT_HD void put(int k, uint32_t n, float l)
+    {
+        node[k * stride + lane] = n;
+        lo[k * stride + lane] = l;
+    }
     MCRT_HD void push(int sp, uint32_t n, float l)
     {
         if (sp < depth)
@@ -284,10 +290,26 @@ struct Tracer
                 const uint32_t first = (uint32_t)__builtin_ctz(m);
                 const int c = __builtin_popcount(m);
                 cur = (p0 && r0 == first) ? ch.x : (p1 && r1 == first) ? ch.y : (p2 && r2 == first) ? ch.z : ch.w;
-                if (p0 && r0 != first) st.push(sp + __builtin_popcount(m >> (r0 + 1u)), ch.x, l0);
-                if (p1 && r1 != first) st.push(sp + __builtin_popcount(m >> (r1 + 1u)), ch.y, l1);
-                if (p2 && r2 != first) st.push(sp + __builtin_popcount(m >> (r2 + 1u)), ch.z, l2);
-                if (p3 && r3 != first) st.push(sp + __builtin_popcount(m >> (r3 + 1u)), ch.w, l3);
+                const bool q0 = p0 && r0 != first, q1 = p1 && r1 != first;
+                const bool q2 = p2 && r2 != first, q3 = p3 && r3 != first;
+                const int i0 = sp + __builtin_popcount(m >> (r0 + 1u)), i1 = sp + __builtin_popcount(m >> (r1 + 1u));
+                const int i2 = sp + __builtin_popcount(m >> (r2 + 1u)), i3 = sp + __builtin_popcount(m >> (r3 + 1u));
+                if (sp + c - 1 <= st.depth)
+                {
+                    // every push lands in LDS: four unconditional writes, those
+                    // of slots not pushed into the discard row
+                    st.put(q0 ? i0 : st.depth, ch.x, l0);
+                    st.put(q1 ? i1 : st.depth, ch.y, l1);
+                    st.put(q2 ? i2 : st.depth, ch.z, l2);
+                    st.put(q3 ? i3 : st.depth, ch.w, l3);
+                }
+                else
+                {
+                    if (q0) st.push(i0, ch.x, l0);
+                    if (q1) st.push(i1, ch.y, l1);
+                    if (q2) st.push(i2, ch.z, l2);
+                    if (q3) st.push(i3, ch.w, l3);
+                }
                 sp += c - 1;
                 return false;
             }

@@ -47,8 +47,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
                                                  unsigned long long* __restrict__ counters, TraceBatch B,
                                                  uint32_t* __restrict__ wq)
 {
-    __shared__ uint32_t s_node[D * kWave];
-    __shared__ float s_lo[D * kWave];
+    __shared__ uint32_t s_node[(D + 1) * kWave];  // + the discard row
+    __shared__ float s_lo[(D + 1) * kWave];
     TraceStack st;
     st.node = s_node;
     st.lo = s_lo;

@@ -128,8 +128,8 @@ extern "C" void emu_trace(void* h, const float* rays, long long n, int* hits, un
     std::atomic<long long> next{0};
     std::vector<long long> cnts(3 * nt, 0);
     auto work = [&](int tid) {
-        std::vector<uint32_t> sn(lds_depth);
-        std::vector<float> sl(lds_depth);
+        std::vector<uint32_t> sn(lds_depth + 1);  // + the discard row
+        std::vector<float> sl(lds_depth + 1);
         int spill_depth = e->wb.stack_bound > lds_depth ? e->wb.stack_bound - lds_depth : 0;
         std::vector<uint64_t> spill(spill_depth + 1);
         TraceStack st;
