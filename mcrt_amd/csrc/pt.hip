@@ -126,21 +126,50 @@ __device__ __forceinline__ void store_ray(float4* rays, int64_t i, V3f o, V3f d,
 }
 __device__ __forceinline__ V3f f3(float4 v) { return V3f{v.x, v.y, v.z}; }
 
-// Slot of this lane in a compaction by the lanes active at the call: one
-// atomic per wavefront (its lowest active lane adds per x the active count)
-// instead of one per lane -- device-scope atomics on one address
-// serialise -- and consecutive slots for consecutive lanes, so the writes
-// that follow coalesce.  Slot order never affects a result: every path's
-// records are its own.
-__device__ __forceinline__ uint32_t wave_slot(uint32_t* counter, uint32_t per)
+// Slots of this thread in the workgroup's two compactions of one
+// iteration: continuation rays (counter c[0], one slot each) and
+// next-event records of `per` rays each (counter c[1], in rays; sb is the
+// record's first ray slot).  One 64-bit atomic on the two adjacent
+// counters per workgroup and iteration: device-scope atomics on one
+// address serialise (about 12 ns each on MI355X), so one per wavefront and
+// counter -- 2^19 for 2^24 paths -- would take over 5 ms by themselves.
+// Consecutive threads get consecutive slots, so the stores that follow
+// coalesce.  Every thread of the workgroup calls it (it synchronises the
+// workgroup).  Slot order never affects a result: every path's records
+// are its own.
+__device__ __forceinline__ void block_slots(uint32_t* c, bool a, bool b, uint32_t per, uint32_t& sa, uint32_t& sb)
 {
-    const uint64_t active = __ballot(1);
-    const uint32_t lane = threadIdx.x & 63u;
-    const int leader = __ffsll((unsigned long long)active) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(active) * per);
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(active & ((1ull << lane) - 1ull)) * per;
+    constexpr uint32_t W = kTPB / 64;
+    __shared__ uint32_t s_n[2][W];
+    __shared__ unsigned long long s_base;
+    const uint64_t ma = __ballot(a), mb = __ballot(b);
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    if (lane == 0)
+    {
+        s_n[0][w] = (uint32_t)__popcll(ma);
+        s_n[1][w] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    uint32_t pa = 0, pb = 0, ta = 0, tb = 0;
+    for (uint32_t q = 0; q < W; q++)
+    {
+        const uint32_t na = s_n[0][q], nb = s_n[1][q];
+        pa += q < w ? na : 0u;
+        pb += q < w ? nb : 0u;
+        ta += na;
+        tb += nb;
+    }
+    // c[0] stays below 2^28 (the largest chunk), so the low word never
+    // carries into c[1]
+    if (threadIdx.x == 0 && (ta | tb) != 0)
+        s_base = atomicAdd((unsigned long long*)c, (unsigned long long)ta | ((unsigned long long)(tb * per) << 32));
+    __syncthreads();
+    // s_n and s_base are rewritten only after the next call's first barrier,
+    // which every thread reaches after these reads
+    const unsigned long long base = s_base;
+    const uint64_t below = (1ull << lane) - 1ull;
+    sa = (uint32_t)base + pa + (uint32_t)__popcll(ma & below);
+    sb = (uint32_t)(base >> 32) + (pb + (uint32_t)__popcll(mb & below)) * per;
 }
 __device__ __forceinline__ float4 f4(V3f v) { return make_float4(v.x, v.y, v.z, 0.0f); }
 
@@ -479,27 +508,64 @@ __device__ __forceinline__ void env_terms(const DevScene& D, uint64_t key, V3f n
 // launch has an environment map (escape radiance, rs = 4 next-event rays);
 // without one the environment code is compiled out, which keeps the
 // kernel's registers, and so its occupancy, those of area lights alone.
+// What a vertex emits -- a next-event record, a continuation ray -- depends
+// only on its material, the parameters and the roulette draw, so it is
+// decided first and the workgroup takes its slots (block_slots) before
+// anything is computed or stored.
 template <bool ADJ, bool BATCH, bool ENV>
 __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
 {
     __shared__ float acc[ADJ ? 2 * kHotFloats : 1];
     if (ADJ) hot_clear(acc);
+    constexpr int rs = ENV ? 4 : 2;
     const int nxt = cur ^ 1;
     const uint32_t A = S.cnt[cur][0];
     const float4* rays = S.rays[cur];
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < A; base += (int64_t)gridDim.x * blockDim.x)
     {
         const int64_t j = base + threadIdx.x;
+        const bool act = j < A;
         int alb_key = -1, em_key = -1, hot = -1;
         V3f g_alb = v3f(0.0f, 0.0f, 0.0f), g_em = g_alb;
+        uint32_t p = 0;
+        int4 h = make_int4(-1, 0, 0, 0);
+        uint16_t sid = 0;
+        if (act)
+        {
+            p = S.act[cur][j];
+            h = S.hits[j];
+            if (BATCH) sid = S.sid[p];
+        }
+        const ScenePT& sp = BATCH ? a.tab[sid] : a.one;
+        const DevScene& D = sp.D;
+        TriShade ts{};
+        DevMaterial mt{};
+        uint64_t key = 0;
+        bool want_rec = false, want_cont = false;
+        if (act && h.x >= 0)
+        {
+            ts = D.shade[h.x];
+            mt = D.mats[ts.mat];
+            if (mt.type != MCRT_LGHT)
+            {
+                if (BATCH)
+                    key = mix64(S.pkey[p] ^ (S.skey[p] | (uint64_t)(uint32_t)k));
+                else
+                {
+                    const int64_t pixel = a.pix0 + p / sp.spp;
+                    const uint64_t sample = (uint64_t)(sp.sample_offset + p % sp.spp);
+                    key = rng_key(sp.seed, (uint64_t)pixel, sample, (uint32_t)k);
+                }
+                want_rec = sp.nee && mt.type == MCRT_DIFF && (D.nlights > 0 || (ENV && D.env_w > 0));
+                // no continuation after the last vertex, or when the roulette ends the path
+                want_cont = !(k >= sp.max_depth) && !(k >= sp.rr_start && rng_u01(key, 7) >= sp.rr_prob);
+            }
+        }
+        uint32_t cslot, rslot;
+        block_slots(S.cnt[nxt], want_cont, want_rec, (uint32_t)rs, cslot, rslot);
         do
         {
-            if (j >= A) break;
-            const uint32_t p = S.act[cur][j];
-            const int4 h = S.hits[j];
-            const uint16_t sid = BATCH ? S.sid[p] : 0;
-            const ScenePT& sp = BATCH ? a.tab[sid] : a.one;
-            const DevScene& D = sp.D;
+            if (!act) break;
             if (h.x < 0)
             {
                 // escape: the passive environment term (demo_pt.cpp:528-540)
@@ -513,8 +579,6 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
             const float4 ra = rays[2 * j], rb = rays[2 * j + 1];
             const V3f o = f3(ra), d = f3(rb);
             const float t = float_of((uint32_t)h.y), u = float_of((uint32_t)h.z), v = float_of((uint32_t)h.w);
-            const TriShade ts = D.shade[h.x];
-            const DevMaterial mt = D.mats[ts.mat];
             const V3f col = ld3(mt.rgb);
             const V3f phit = o + d * t;
             const V3f ns = shading_normal(ts, u, v);
@@ -547,21 +611,10 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                             col.z > 0.0f ? dl.z * rest.z / col.z : 0.0f);
             }
             S.last_diff[p] = mt.type == MCRT_DIFF;
-            uint64_t key;
-            if (BATCH)
-                key = mix64(S.pkey[p] ^ (S.skey[p] | (uint64_t)(uint32_t)k));
-            else
+            if (want_rec)
             {
-                const int64_t pixel = a.pix0 + p / sp.spp;
-                const uint64_t sample = (uint64_t)(sp.sample_offset + p % sp.spp);
-                key = rng_key(sp.seed, (uint64_t)pixel, sample, (uint32_t)k);
-            }
-            if (sp.nee && mt.type == MCRT_DIFF && (D.nlights > 0 || (ENV && D.env_w > 0)))
-            {
-                constexpr int rs = ENV ? 4 : 2;
-                const uint32_t slot = wave_slot(&S.cnt[nxt][1], (uint32_t)rs);
-                const int64_t m = slot / (uint32_t)rs;
-                const int64_t r0i = a.P + rs * m;
+                const int64_t m = rslot / (uint32_t)rs;
+                const int64_t r0i = a.P + (int64_t)rslot;
                 float4* R = S.rec[nxt] + rec_words(rs) * m;
                 float4* nr = S.rays[nxt];
                 const V3f zero = v3f(0.0f, 0.0f, 0.0f);
@@ -617,6 +670,7 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 if (BATCH)
                     for (int i = 0; i < rs; i++) S.rsid[nxt][r0i + i] = sid;
             }
+            if (!want_cont) break;
             // Continuation (Material::sample, demo_pt.cpp:645-650)
             const V3f wi = -d;
             V3f nd, weight;
@@ -632,17 +686,11 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 weight = col;
             }
             T = T * weight;
-            if (k >= sp.max_depth) break;
-            if (k >= sp.rr_start)
-            {
-                if (rng_u01(key, 7) >= sp.rr_prob) break;
-                T = T * (1.0f / sp.rr_prob);
-            }
+            if (k >= sp.rr_start) T = T * (1.0f / sp.rr_prob);
             S.T[p] = f4(T);
-            const uint32_t slot = wave_slot(&S.cnt[nxt][0], 1u);
-            store_ray(S.rays[nxt], slot, phit, nd, kEpsRay, __builtin_inff());
-            S.act[nxt][slot] = p;
-            if (BATCH) S.rsid[nxt][slot] = sid;
+            store_ray(S.rays[nxt], cslot, phit, nd, kEpsRay, __builtin_inff());
+            S.act[nxt][cslot] = p;
+            if (BATCH) S.rsid[nxt][cslot] = sid;
         } while (false);
         if (ADJ)
         {
