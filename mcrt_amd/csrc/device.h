@@ -56,7 +56,13 @@ struct DevScene
     const float* env_cdf_v = nullptr;  // CDF over the rows of each column (env_w x env_h, column-major)
     int32_t env_w = 0, env_h = 0;
     float env_norm = 0.0f;             // 1 / sum of the sin-warped luminance
-    int32_t pad1 = 0;
+    // Union of the boxes of every reference to an emitter (LGHT) triangle
+    // (emit_any == 0: there is none).  A ray that fails the traversal's slab
+    // test against it fails it against every emitter reference, so its
+    // closest hit cannot be an emitter.
+    int32_t emit_any = 0;
+    float emit_lo[3] = {0.0f, 0.0f, 0.0f};
+    float emit_hi[3] = {0.0f, 0.0f, 0.0f};
 };
 
 // The part of a scene a traversal reads (batched launches look it up per ray).

@@ -654,9 +654,25 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 R[1] = make_float4(col.x, col.y, col.z, factor2);
                 R[2] = make_float4(wb.x, wb.y, wb.z, float_of(p));
                 R[3] = make_float4(wo.x, wo.y, wo.z, float_of((uint32_t)li_rec));
+                // The BSDF-sampled ray adds a term only when its closest hit is
+                // an emitter, and the traversal accepts a triangle only after its
+                // reference's box passes the slab test.  That test is monotone in
+                // the box (round-to-nearest keeps (b - o) * inv_dir ordered in b),
+                // so a ray failing it against the box around every emitter
+                // reference (with tmax = inf) reaches none, and its terms are a
+                // miss's: it is not traced.  Rays with an infinite inv_dir take
+                // the reference's NaN-prone sequential test and are always traced.
+                bool trace_wb = false;
+                if (D.emit_any)
+                {
+                    const RayPrep rw = prep_ray(phit.x, phit.y, phit.z, wb.x, wb.y, wb.z, kEpsRay);
+                    float lo;
+                    trace_wb = rw.exact || slab(rw, D.emit_lo[0], D.emit_hi[0], D.emit_lo[1], D.emit_hi[1],
+                                                D.emit_lo[2], D.emit_hi[2], __builtin_inff(), lo);
+                }
                 // a zero direction fails every slab test (ray.cpp:77-79): the
                 // rays a vertex does not need cost one root visit
-                store_ray(nr, r0i, phit, wb, kEpsRay, __builtin_inff());
+                store_ray(nr, r0i, phit, trace_wb ? wb : zero, kEpsRay, __builtin_inff());
                 store_ray(nr, r0i + 1, phit, li_rec >= 0 ? wo : zero, kEpsRay, __builtin_inff());
                 if (ENV)
                 {

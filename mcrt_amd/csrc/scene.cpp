@@ -149,6 +149,44 @@ uint64_t triangles_hash(const std::vector<Tri>& tris)
     return hash_bytes(tris.data(), tris.size() * sizeof(Tri), 0xcbf29ce484222325ull ^ (uint64_t)tris.size());
 }
 
+// Structure of a loaded payload: every primitive id below ntri, every leaf
+// link inside the reference array, every interior link after its parent
+// (the order collapse_wide writes) and inside the node array, and the
+// recorded stack bound at least the one the links imply (collapse_wide's
+// recursion: a node pushes its passing children and visits one).  The
+// hashes catch accidental damage; this keeps a file that passes them but
+// was written by something else from sending the traversal outside its
+// arrays or beyond its stack.
+bool cache_structure_valid(const WideBVH& wb, uint64_t ntri, int32_t stack_bound)
+{
+    const size_t nq = wb.qnodes.size(), nr = wb.refs.size();
+    for (const WRef& r : wb.refs)
+        if (r.prim < 0 || (uint64_t)r.prim >= ntri) return false;
+    std::vector<int32_t> need(nq, 0);
+    for (size_t i = nq; i-- > 0;)
+    {
+        const QNode& q = wb.qnodes[i];
+        int32_t nchild = 0, child_need = 0;
+        for (int c = 0; c < 4; c++)
+        {
+            const uint32_t l = q.child[c];
+            if (l == MCRT_CHILD_EMPTY) continue;
+            nchild++;
+            if (l & MCRT_CHILD_LEAF)
+            {
+                if ((uint64_t)(l & MCRT_LEAF_REF_MASK) + ((l >> 29) & 3u) + 1u > nr) return false;
+            }
+            else
+            {
+                if (l <= i || l >= nq) return false;
+                child_need = std::max(child_need, need[l]);
+            }
+        }
+        need[i] = std::max(nchild, nchild - 1 + child_need);
+    }
+    return nq == 0 || need[0] <= stack_bound;
+}
+
 bool load_bvh_cache(const char* path, uint64_t ntri, uint64_t tri_hash, WideBVH& wb, mcrt_build_stats& st)
 {
     FILE* f = std::fopen(path, "rb");
@@ -172,6 +210,7 @@ bool load_bvh_cache(const char* path, uint64_t ntri, uint64_t tri_hash, WideBVH&
              std::fread(wb.refs.data(), sizeof(WRef), h.nr, f) == h.nr;
         ok = ok && hash_bytes(wb.refs.data(), h.nr * sizeof(WRef),
                               hash_bytes(wb.qnodes.data(), h.nq * sizeof(QNode))) == h.payload_hash;
+        ok = ok && cache_structure_valid(wb, ntri, h.stack_bound);
     }
     std::fclose(f);
     if (!ok)
@@ -332,6 +371,21 @@ int finalize_scene(mcrt_scene* s, int threads, const char* cache)
         }
     }
 
+    // Box around every emitter reference (the clipped boxes the traversal
+    // tests before Moller-Trumbore), for the path tracer's next-event cull.
+    bool emit_any = false;
+    float emit_lo[3] = {0.0f, 0.0f, 0.0f}, emit_hi[3] = {0.0f, 0.0f, 0.0f};
+    for (const WRef& r : wb.refs)
+    {
+        if (s->mats[s->tri_mat[r.prim]].type != MCRT_LGHT) continue;
+        for (int k = 0; k < 3; k++)
+        {
+            emit_lo[k] = emit_any ? std::min(emit_lo[k], r.lo[k]) : r.lo[k];
+            emit_hi[k] = emit_any ? std::max(emit_hi[k], r.hi[k]) : r.hi[k];
+        }
+        emit_any = true;
+    }
+
     int rc;
     if ((rc = s->d_nodes.upload(wb.qnodes.data(), wb.qnodes.size() * sizeof(QNode)))) return rc;
     if ((rc = s->d_refs.upload(wb.refs.data(), wb.refs.size() * sizeof(WRef)))) return rc;
@@ -352,6 +406,12 @@ int finalize_scene(mcrt_scene* s, int threads, const char* cache)
     d.nhot = (int32_t)hot_tri.size();
     d.stack_bound = wb.stack_bound;
     d.ntri = (int64_t)n;
+    d.emit_any = emit_any ? 1 : 0;
+    for (int k = 0; k < 3; k++)
+    {
+        d.emit_lo[k] = emit_lo[k];
+        d.emit_hi[k] = emit_hi[k];
+    }
     apply_trace_tuning(s);
     auto t2 = clk::now();
     st.build_seconds = std::chrono::duration<double>(t1 - t0).count();

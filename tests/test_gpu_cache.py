@@ -64,6 +64,85 @@ def test_bvh_cache_rejects_other_triangles_and_damage(ctx, tmp_path):
     assert e.to_scene(ctx, bvh_cache=path).stats["from_cache"] == 0
 
 
+_HDR = 112  # CacheHeader bytes: payload_hash at 104, stack_bound at 56, nq at 40, nr at 48
+
+
+def _fnv_words(b, h):
+    """The cache's payload hash: FNV-1a over little-endian 64-bit words, then
+    the remaining bytes."""
+    m = (1 << 64) - 1
+    n8 = len(b) // 8 * 8
+    for w in np.frombuffer(bytes(b[:n8]), dtype="<u8").tolist():
+        h = ((h ^ w) * 0x100000001b3) & m
+    for x in b[n8:]:
+        h = ((h ^ x) * 0x100000001b3) & m
+    return h
+
+
+def _rehash(raw):
+    nq, nr = np.frombuffer(bytes(raw[40:56]), dtype="<u8").tolist()
+    q = raw[_HDR:_HDR + 64 * nq]
+    r = raw[_HDR + 64 * nq:_HDR + 64 * (nq + nr)]
+    h = _fnv_words(r, _fnv_words(q, 0xcbf29ce484222325))
+    raw[104:112] = np.array([h], dtype="<u8").tobytes()
+    return nq, nr
+
+
+def test_bvh_cache_rejects_consistent_files_with_bad_structure(ctx, tmp_path):
+    """A file whose hashes match its payload but whose links, primitive ids or
+    stack bound do not fit the arrays is rebuilt, not loaded."""
+    path = str(tmp_path / "scene.mcrtbvh")
+    d = _data(freq=8)
+    ref = d.to_scene(ctx, bvh_cache=path)
+    good = bytearray(open(path, "rb").read())
+    nq, nr = _rehash(bytearray(good))
+    assert d.to_scene(ctx, bvh_cache=path).stats["from_cache"] == 1
+    # the rehash reproduces the writer's hash, so only the edits below differ
+    chk = bytearray(good)
+    _rehash(chk)
+    assert chk == good
+    rays = np.random.default_rng(6).uniform(1.0, 54.0, (2048, 8)).astype(np.float32)
+    rays[:, 3] = 1e-4
+    rays[:, 4:7] /= np.linalg.norm(rays[:, 4:7], axis=1, keepdims=True)
+    rays[:, 7] = np.inf
+    want = ref.intersect(rays)
+
+    def first_link(raw, leaf):
+        for i in range(nq):
+            for c in range(4):
+                off = _HDR + 64 * i + 16 + 4 * c
+                link = int(np.frombuffer(bytes(raw[off:off + 4]), dtype="<u4")[0])
+                if link != 0xFFFFFFFF and bool(link & 0x80000000) == leaf:
+                    return off, link
+        raise AssertionError("no such link")
+
+    edits = {}
+    raw = bytearray(good)  # a primitive id past the triangles
+    off = _HDR + 64 * nq + 60
+    raw[off:off + 4] = np.array([len(d.tris) + 5], dtype="<i4").tobytes()
+    edits["prim"] = raw
+    raw = bytearray(good)  # an interior link back to the root (a cycle)
+    off, _ = first_link(raw, leaf=False)
+    raw[off:off + 4] = np.array([0], dtype="<u4").tobytes()
+    edits["cycle"] = raw
+    raw = bytearray(good)  # a leaf link past the references
+    off, link = first_link(raw, leaf=True)
+    raw[off:off + 4] = np.array([(link & 0xE0000000) | (nr - 1)], dtype="<u4").tobytes()
+    if ((link >> 29) & 3) == 0:
+        raw[off:off + 4] = np.array([0xA0000000 | (nr - 1)], dtype="<u4").tobytes()
+    edits["leaf range"] = raw
+    raw = bytearray(good)  # a stack bound below what the tree needs
+    sb = int(np.frombuffer(bytes(raw[56:60]), dtype="<i4")[0])
+    raw[56:60] = np.array([sb - 1], dtype="<i4").tobytes()
+    edits["stack bound"] = raw
+    for name, raw in edits.items():
+        _rehash(raw)
+        open(path, "wb").write(bytes(raw))
+        s = d.to_scene(ctx, bvh_cache=path)
+        assert s.stats["from_cache"] == 0, name
+        assert np.array_equal(s.intersect(rays), want), name
+
+
 def test_bvh_cache_unwritable_path(ctx, tmp_path):
     import mcrt_amd
     with pytest.raises(mcrt_amd.McrtError) as e:
