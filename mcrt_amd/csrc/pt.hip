@@ -433,7 +433,6 @@ __global__ void k_resolve(PTArgs a, PTState S, int cur)
             V3f local = v3f(0.0f, 0.0f, 0.0f);
             if (ENV)
             {
-                // these two rays were traced any-hit: only a miss is read
                 const V3f z = v3f(0.0f, 0.0f, 0.0f);
                 const V3f e1 = S.hits[h0 + 2].x < 0 ? f3(Rm[4]) : z;
                 const V3f e2 = S.hits[h0 + 3].x < 0 ? f3(Rm[5]) : z;
@@ -835,10 +834,6 @@ void bounce_loop(const DevScene& Dt, PTState& S, const PTArgs& a, int max_depth,
         TraceBatch tb;
         tb.roots = a.roots;
         tb.rsid = S.rsid[cur];
-        // the environment samples' shadow rays (slots 2, 3) count only when
-        // they escape
-        tb.slot_mask = (uint32_t)a.rs - 1u;
-        tb.any_slots = a.rs == 4 ? 0xcu : 0u;
         launch_k_trace<false, false, true, BATCH>(Dt, trace_blocks(Dt, (1 + a.rs) * a.P), st, S.rays[cur], (int64_t)0,
                                                   S.cnt[cur], a.P, S.hits, (uint8_t*)nullptr,
                                                   (unsigned long long*)nullptr, S.cnt[cur] + kCntWq, tb);

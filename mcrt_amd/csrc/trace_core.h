@@ -201,9 +201,7 @@ struct TraceCount
 // The traversal of one ray as a sequence of steps (one wide node or one
 // leaf per step), so a kernel can hand a lane its next ray as soon as the
 // current one ends instead of waiting for the wavefront's slowest lane.
-// MIXED: each ray says at start whether it ends at its first accepted hit
-// (any-hit) or runs to the closest.
-template <bool ANY, bool COUNT, bool MIXED = false>
+template <bool ANY, bool COUNT>
 struct Tracer
 {
     RayPrep r;
@@ -211,9 +209,8 @@ struct Tracer
     int prim;
     uint32_t cur;
     int sp;
-    bool any_ray = false;  // MIXED: this ray stops at its first accepted hit
 
-    MCRT_HD void start(const RayPrep& rp, float tmax, bool any_hit = false)
+    MCRT_HD void start(const RayPrep& rp, float tmax)
     {
         r = rp;
         R1 = tmax;
@@ -221,7 +218,6 @@ struct Tracer
         prim = -1;
         cur = 0;
         sp = 0;
-        any_ray = any_hit;
     }
 
     // The 64 B record `cur` names: a QNode, or a leaf reference.  Sc is any
@@ -340,7 +336,7 @@ struct Tracer
                     bu = u;
                     bv = v;
                     prim = (int)bits_of(d.w);
-                    if (ANY || (MIXED && any_ray)) return true;
+                    if (ANY) return true;
                 }
             }
             if (count > 1)

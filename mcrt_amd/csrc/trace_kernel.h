@@ -21,11 +21,7 @@
 //     next-event rays [seg1, seg1 + cnt[1])), so the bounce loop needs no
 //     host round trip and can be captured in a graph.
 // BATCH = true: the rays belong to several scenes (batched rendering); ray
-// i is traversed in the BVH of scene rsid[i].  With SEG, the second
-// segment holds records of (slot_mask + 1) rays, and a ray whose slot's bit
-// is set in any_slots stops at its first accepted hit: its caller reads
-// only whether it hit, which any-hit answers exactly (up to the first
-// accepted hit both traversals make the same steps).
+// i is traversed in the BVH of scene rsid[i].
 #pragma once
 #include "trace_core.h"
 
@@ -34,14 +30,11 @@ namespace mcrt
 constexpr int kWave = 64;
 constexpr uint32_t kClaim = 128;
 
-// Rays of several scenes in one launch: ray i belongs to scene rsid[i]
-// (BATCH); any-hit slots of the second segment's records (SEG).
+// Rays of several scenes in one launch: ray i belongs to scene rsid[i].
 struct TraceBatch
 {
     const SceneRoot* roots = nullptr;
     const uint16_t* rsid = nullptr;
-    uint32_t any_slots = 0;  // bit s: slot s of a record is any-hit
-    uint32_t slot_mask = 0;  // rays per record - 1 (a power of two)
 };
 
 // With an 8-entry stack, eight waves per SIMD (32 per CU) fit in LDS; they
@@ -78,7 +71,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
     uint32_t nrays = 0;
     const uint32_t lane = threadIdx.x;
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64u - lane));  // lanes under this one
-    Tracer<ANY, COUNT, SEG> tr;
+    Tracer<ANY, COUNT> tr;
     SceneRoot root;
     root.nodes = S.nodes;
     root.refs = S.refs;
@@ -132,8 +125,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
                 i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : k;
                 if (BATCH) root = B.roots[B.rsid[i]];
                 const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
-                tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w,
-                         SEG && k >= n0 && ((B.any_slots >> ((k - n0) & B.slot_mask)) & 1u));
+                tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w);
                 live = true;
             }
             const uint32_t nidle = (uint32_t)__popcll(idle);
