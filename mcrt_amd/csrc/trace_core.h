@@ -93,9 +93,13 @@ MCRT_HD bool slab(const RayPrep& r, float lx, float hx, float ly, float hy, floa
 }
 
 // A QNode plane (layout.h: qnode_plane): origin + q * 2^k, q one byte.
+// The product q * 2^k is exact (q < 256 needs 8 significant bits, and a
+// normal 2^k keeps it normal), so the fused multiply-add rounds once, in
+// the addition, and equals layout.h's origin + q * scale bit for bit in one
+// instruction instead of two.
 MCRT_HD float qplane(float origin, uint32_t word, int shift, float scale)
 {
-    return origin + (float)((word >> shift) & 0xffu) * scale;
+    return __builtin_fmaf((float)((word >> shift) & 0xffu), scale, origin);
 }
 
 // Slab test of child `shift / 8` of a QNode against its decoded

@@ -48,6 +48,32 @@ def test_render_bit_exact_glossy_per_triangle(ctx):
     _compare("glossy", g, o)
 
 
+def test_render_bit_exact_two_separated_lights(ctx):
+    """Two emitters far apart (the ceiling light and a patch low on the green
+    wall), glossy triangles and roulette: the box around the emitter
+    references that culls BSDF next-event rays spans both lights and the
+    empty space between them, and the film stays bit-identical."""
+    from mcrt_amd import api, scenes
+    d = scenes.cornell_displaced(sub=3, phong_frac=0.4)
+    x, y0, y1, z0, z1 = 0.3, 5.0, 15.0, 10.0, 25.0
+    patch = np.array([[(x, y0, z1), (x, y0, z0), (x, y1, z0)],
+                      [(x, y0, z1), (x, y1, z0), (x, y1, z1)]], np.float32)
+    pn = np.repeat(scenes._face_normals(patch)[:, None, :], 3, axis=1)
+    assert (pn[:, 0, 0] > 0.99).all()  # facing into the box
+    m = len(d.mat_type)
+    e = scenes.SceneData(np.concatenate([d.tris.reshape(-1, 3, 3), patch]),
+                         np.concatenate([d.normals.reshape(-1, 3, 3), pn]),
+                         np.concatenate([d.tri_mat, [m, m]]), np.concatenate([d.mat_type, [api.LGHT]]),
+                         np.concatenate([d.mat_rgb, [(9.0, 5.0, 2.0)]]), np.concatenate([d.mat_gloss, [25.0]]))
+    for nee in (True, False):
+        kw = dict(spp=3, max_depth=6, rr_start=3, rr_prob=0.8, seed=13, nee=nee)
+        print(f"[two lights nee={nee}]", flush=True)
+        g = gpu_render(ctx, e, 40, 32, **kw)
+        o = oracle_render(e, 40, 32, **kw)
+        assert g.mean() > 0.01
+        _compare(f"two lights nee={nee}", g, o)
+
+
 def test_render_single_spp_no_filter(ctx):
     from mcrt_amd import scenes
     d = scenes.cornell_icosphere(1)
