@@ -120,16 +120,6 @@ MCRT_HD bool qslab_nf(const RayPrep& r, uint32_t child, uint32_t nx, uint32_t fx
     return (lo <= hi) & (lo <= R1) & (hi >= r.R0) & (child != MCRT_CHILD_EMPTY);
 }
 
-// Slab test of child `shift / 8` of a QNode against its decoded
-// (outward-rounded) box.
-MCRT_HD bool qslab(const RayPrep& r, uint32_t child, const uint4& qa, const uint4& qb, float ox, float oy, float oz,
-                   float sx, float sy, float sz, int shift, float R1, float& lo)
-{
-    if (child == MCRT_CHILD_EMPTY) return false;
-    return slab(r, qplane(ox, qa.x, shift, sx), qplane(ox, qa.y, shift, sx), qplane(oy, qa.z, shift, sy),
-                qplane(oy, qa.w, shift, sy), qplane(oz, qb.x, shift, sz), qplane(oz, qb.y, shift, sz), R1, lo);
-}
-
 // rayTriangleIntersect (triangle.cpp:13-51), e1/e2 precomputed.
 MCRT_HD bool mt(const RayPrep& r, float v0x, float v0y, float v0z, float e1x, float e1y,
                                    float e1z, float e2x, float e2y, float e2z, float& t, float& u, float& v)
