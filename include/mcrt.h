@@ -253,6 +253,13 @@ typedef struct mcrt_tuning {
                                       launches of renders, whose ray sets (many
                                       short shadow rays) favour more waves: (32) */
     int32_t pt_trace_lds_depth;    /* (8)                                         */
+    int64_t trace_reorder_rays;    /* a query of at least this many rays is
+                                      traversed in the order of its rays' origin
+                                      cells (16^3 over the scene's box, a counting
+                                      sort on the device; results unchanged);
+                                      -1: never.  Unset, finalize picks 2^20 when
+                                      the BVH exceeds 192 MB (larger than the GPU's
+                                      last-level cache), else -1                 */
 } mcrt_tuning;
 MCRT_API int mcrt_scene_set_tuning(mcrt_scene* scene, const mcrt_tuning* tuning);
 MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* scene, mcrt_tuning* tuning);

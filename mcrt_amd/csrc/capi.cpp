@@ -416,7 +416,7 @@ MCRT_API int mcrt_query_counters(mcrt_scene* s, const float* rays, size_t n, int
     StreamRes* res = stream_res(s, st);
     std::lock_guard<std::mutex> g(res->mu);
     DevScene view;
-    if ((rc = launch_view(s, res, view))) return rc;
+    if ((rc = launch_view(s, res, view, (int64_t)n))) return rc;
     HIPCHK(launch_trace(view, r, (int64_t)n, any_hit ? nullptr : d_out.p, any_hit ? d_out.as<uint8_t>() : nullptr,
                         any_hit ? TRACE_ANY : TRACE_CLOSEST, d_cnt.as<uint64_t>(), st));
     HIPCHK(hipMemcpyAsync(counters, d_cnt.p, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
@@ -436,6 +436,8 @@ static int check_tuning(const mcrt_tuning* t)
         return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: pt_trace_waves_per_cu in [1, 1024] (0 = keep)");
     if (t->pt_trace_lds_depth != 0 && !valid_lds_depth(t->pt_trace_lds_depth))
         return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: pt_trace_lds_depth must be 8, 12, 16 or 24 (0 = keep)");
+    if (t->trace_reorder_rays < -1)
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_reorder_rays is -1 (never) or a ray count (0 = keep)");
     return MCRT_OK;
 }
 
@@ -449,6 +451,7 @@ MCRT_API int mcrt_scene_set_tuning(mcrt_scene* s, const mcrt_tuning* t)
     if (t->pt_chunk_paths) s->pt_chunk_paths = t->pt_chunk_paths;
     if (t->pt_trace_waves_per_cu) s->pt_trace_waves_per_cu = t->pt_trace_waves_per_cu;
     if (t->pt_trace_lds_depth) s->pt_trace_lds_depth = t->pt_trace_lds_depth;
+    if (t->trace_reorder_rays) s->trace_reorder_rays = t->trace_reorder_rays;
     if (s->finalized) apply_trace_tuning(s);
     return MCRT_OK;
 }
@@ -461,6 +464,7 @@ MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* s, mcrt_tuning* t)
     t->pt_chunk_paths = s->pt_chunk_paths;
     t->pt_trace_waves_per_cu = s->pt_trace_waves_per_cu;
     t->pt_trace_lds_depth = s->pt_trace_lds_depth;
+    t->trace_reorder_rays = s->trace_reorder_rays;
     return MCRT_OK;
 }
 
@@ -507,7 +511,7 @@ MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcr
                 s->trace_lds_depth = d;
                 apply_trace_tuning(s);
                 DevScene view;
-                if ((rc = launch_view(s, res, view))) return rc;
+                if ((rc = launch_view(s, res, view, (int64_t)n))) return rc;
                 HIPCHK(launch_trace(view, r, (int64_t)n, d_hits.p, nullptr, TRACE_CLOSEST, nullptr, st));
                 HIPCHK(hipEventRecord(e0, st));
                 for (int k = 0; k < 3; k++)

@@ -87,6 +87,7 @@ struct StreamRes
     std::mutex mu;
     DevBuf spill;
     DevBuf wq;  // ray-query work counter
+    DevBuf order, cells;  // traversal order of large queries (DevScene::order, cell_work)
     PTWork ptw;
 };
 }  // namespace mcrt
@@ -137,6 +138,9 @@ struct mcrt_scene
     int32_t pt_trace_waves_per_cu = 32;
     int32_t pt_trace_lds_depth = 8;
     int64_t pt_chunk_paths = int64_t(1) << 24;
+    // queries of at least this many rays are traversed in origin-cell order;
+    // -1: never; 0: not set, decided at finalize by the BVH's size
+    int64_t trace_reorder_rays = 0;
 };
 
 namespace mcrt
@@ -158,8 +162,10 @@ int run_query(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, uint8_
 StreamRes* stream_res(mcrt_scene* s, hipStream_t stream);
 // Sets the traversal tuning of s->dev (LDS depth, spill geometry).
 void apply_trace_tuning(mcrt_scene* s);
-// The scene's device view for a launch with r's spill stacks (caller holds r->mu).
-int launch_view(mcrt_scene* s, StreamRes* r, DevScene& view);
+// The scene's device view for a launch with r's spill stacks (caller holds
+// r->mu); for a query of n rays also r's reorder buffers when n reaches the
+// scene's reorder threshold.
+int launch_view(mcrt_scene* s, StreamRes* r, DevScene& view, int64_t n = 0);
 // The same for the traversal launches of renders (pt_trace_* tuning).
 int render_view(mcrt_scene* s, StreamRes* r, DevScene& view);
 // Sets the traversal geometry of `view` (LDS depth, spill lanes and depth).

@@ -56,6 +56,16 @@ struct DevScene
     const float* env_cdf_v = nullptr;  // CDF over the rows of each column (env_w x env_h, column-major)
     int32_t env_w = 0, env_h = 0;
     float env_norm = 0.0f;             // 1 / sum of the sin-warped luminance
+    // Ray order of large queries (traverse.hip): a query of at least
+    // reorder_rays rays (0: never) is traversed in the order of its rays'
+    // origin cells, a kCellsPerAxis^3 grid over the scene's box (cell_lo,
+    // and cells per unit length in cell_sc).  order (a permutation of the
+    // rays) and cell_work (kCellWorkWords) are per-stream buffers.
+    int64_t reorder_rays = 0;
+    float cell_lo[3] = {0.0f, 0.0f, 0.0f};
+    float cell_sc[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t* order = nullptr;
+    uint32_t* cell_work = nullptr;
     // Union of the boxes of every reference to an emitter (LGHT) triangle
     // (emit_any == 0: there is none).  A ray that fails the traversal's slab
     // test against it fails it against every emitter reference, so its
@@ -77,6 +87,19 @@ struct SceneRoot
 constexpr uint32_t kSlices = 8;
 constexpr uint32_t kSliceStride = 16;  // uint32 words between slice counters
 constexpr uint32_t kWqWords = kSlices * kSliceStride;
+
+// Origin cells of the query reorder: 16 per axis (Morton-ordered), counted
+// and scattered by kCellBlocks workgroups.  cell_work holds the workgroups'
+// offsets inside each cell ([kCellBlocks][kCells]), then the cell totals
+// and the cell bases (kCells each), then a flag set when one cell holds
+// every ray (the order would not change: the reorder is skipped).
+constexpr uint32_t kCellsPerAxis = 16;
+constexpr uint32_t kCells = kCellsPerAxis * kCellsPerAxis * kCellsPerAxis;
+constexpr uint32_t kCellBlocks = 256;
+constexpr size_t kCellTotals = (size_t)kCellBlocks * kCells;
+constexpr size_t kCellBases = kCellTotals + kCells;
+constexpr size_t kCellSkip = kCellBases + kCells;
+constexpr size_t kCellWorkWords = kCellSkip + 64;
 
 // Traversal modes.
 enum TraceMode

@@ -30,17 +30,21 @@ namespace mcrt
 constexpr int kWave = 64;
 constexpr uint32_t kClaim = 128;
 
-// Rays of several scenes in one launch: ray i belongs to scene rsid[i].
+// Rays of several scenes in one launch: ray i belongs to scene rsid[i]
+// (BATCH).  ORDER (queries): unless *skip is set at launch, the k-th ray
+// claimed is rays[order[k]], order a permutation of the batch.
 struct TraceBatch
 {
     const SceneRoot* roots = nullptr;
     const uint16_t* rsid = nullptr;
+    const uint32_t* order = nullptr;
+    const uint32_t* skip = nullptr;
 };
 
 // With an 8-entry stack, eight waves per SIMD (32 per CU) fit in LDS; they
 // also need at most 64 VGPRs and 96 SGPRs, which the attribute asks for.
 // Deeper stacks are held to fewer waves by LDS.
-template <int D, bool ANY, bool COUNT, bool SEG, bool BATCH>
+template <int D, bool ANY, bool COUNT, bool SEG, bool BATCH, bool ORDER>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 8 : 4, 8))) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
                                                  const uint32_t* __restrict__ seg_cnt, int64_t seg1,
                                                  int4* __restrict__ hits, uint8_t* __restrict__ occ,
@@ -76,6 +80,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
     root.nodes = S.nodes;
     root.refs = S.refs;
     uint32_t i = 0;
+    const uint32_t* order = ORDER && *B.skip == 0 ? B.order : nullptr;
     uint32_t claim = 0, claim_end = 0;  // wave-uniform: indices [claim, claim_end) not yet started
     bool drained = false;               // wave-uniform: every slice is used up
     bool live = false;
@@ -122,7 +127,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
             if (!live && rank < avail)
             {
                 const uint32_t k = claim + rank;
-                i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : k;
+                i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : (ORDER && order != nullptr) ? order[k] : k;
                 if (BATCH) root = B.roots[B.rsid[i]];
                 const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
                 tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w);
@@ -167,7 +172,7 @@ inline int64_t trace_blocks(const DevScene& s, int64_t n)
 
 // Launches k_trace with the scene's LDS stack depth on `blocks` workgroups;
 // wq holds kWqWords words, the slice counters zero.
-template <bool ANY, bool COUNT, bool SEG, bool BATCH = false>
+template <bool ANY, bool COUNT, bool SEG, bool BATCH = false, bool ORDER = false>
 inline void launch_k_trace(const DevScene& S, int64_t blocks, hipStream_t st, const float4* rays, int64_t n,
                            const uint32_t* seg_cnt, int64_t seg1, int4* hits, uint8_t* occ,
                            unsigned long long* counters, uint32_t* wq, TraceBatch B = TraceBatch())
@@ -176,19 +181,19 @@ inline void launch_k_trace(const DevScene& S, int64_t blocks, hipStream_t st, co
     switch (S.lds_depth)
     {
     case 8:
-        hipLaunchKernelGGL((k_trace<8, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+        hipLaunchKernelGGL((k_trace<8, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
                            counters, B, wq);
         break;
     case 12:
-        hipLaunchKernelGGL((k_trace<12, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+        hipLaunchKernelGGL((k_trace<12, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
                            counters, B, wq);
         break;
     case 24:
-        hipLaunchKernelGGL((k_trace<24, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+        hipLaunchKernelGGL((k_trace<24, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
                            counters, B, wq);
         break;
     default:
-        hipLaunchKernelGGL((k_trace<16, ANY, COUNT, SEG, BATCH>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+        hipLaunchKernelGGL((k_trace<16, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
                            counters, B, wq);
         break;
     }

@@ -41,6 +41,51 @@ def test_traversal_tuning_bit_exact(ctx, depth, waves):
     assert np.array_equal(s.occluded(r2), (p2 >= 0).astype(np.uint8))
 
 
+@pytest.mark.parametrize("n", [1, 777, (1 << 16) + 13])
+def test_query_reorder_bit_exact(ctx, n):
+    """Queries traversed in the order of their origins' cells
+    (trace_reorder_rays) give the hits of the caller's order bit for bit,
+    closest and any hit, through host and device pointers, with origins
+    inside and far outside the scene's box and a NaN origin; the counters
+    of the traversal are the same sums."""
+    import torch
+    import mcrt_amd
+    print(f"[reorder] n={n}", flush=True)
+    tris, s, _ = _debug_scene(ctx)
+    assert s.tuning()["trace_reorder_rays"] == -1  # a small BVH is not reordered by default
+    rays = _libs.random_rays(tris, n, seed=17)
+    if n > 10:
+        rays[:5, 0:3] *= 1000.0
+    r2 = rays.copy()
+    r2[:, 7] = 20.0
+    ref_hits, ref_occ, ref_cnt = s.intersect(rays), s.occluded(r2), s.query_counters(rays)
+    op, otuv = _libs.OracleBVH(tris).intersect(rays)
+    prim, tuv = mcrt_amd.split_hits(ref_hits)
+    assert np.array_equal(prim, op) and np.array_equal(tuv.view(np.uint32), otuv.view(np.uint32))
+    s.set_tuning(trace_reorder_rays=1)
+    assert s.tuning()["trace_reorder_rays"] == 1
+    assert np.array_equal(s.intersect(rays), ref_hits)
+    assert np.array_equal(s.occluded(r2), ref_occ)
+    assert s.query_counters(rays) == ref_cnt
+    dr = torch.from_numpy(rays).cuda()
+    h = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+    s.intersect(dr, h)
+    torch.cuda.synchronize()
+    assert np.array_equal(h.cpu().numpy(), ref_hits)
+    if n > 10:
+        nan = rays.copy()
+        nan[7, 1] = np.nan
+        cam = rays.copy()  # one origin: the sampled cells agree and the order is kept
+        cam[:, 0:3] = rays[n // 2, 0:3]
+        s.set_tuning(trace_reorder_rays=-1)
+        want, want_cam = s.intersect(nan), s.intersect(cam)
+        s.set_tuning(trace_reorder_rays=1)
+        assert np.array_equal(s.intersect(nan), want)
+        assert np.array_equal(s.intersect(cam), want_cam)
+    s.set_tuning(trace_reorder_rays=-1)
+    assert s.tuning()["trace_reorder_rays"] == -1
+
+
 def test_render_invariant_under_tuning(ctx):
     from mcrt_amd import scenes
     d = scenes.cornell_icosphere(3)
@@ -80,7 +125,8 @@ def test_tuning_errors(ctx):
     import mcrt_amd
     _, s, _ = _debug_scene(ctx)
     for kw in (dict(trace_lds_depth=10), dict(trace_waves_per_cu=1025), dict(trace_waves_per_cu=-1),
-               dict(pt_chunk_paths=(1 << 28) + 1), dict(pt_trace_lds_depth=4), dict(pt_trace_waves_per_cu=2000)):
+               dict(pt_chunk_paths=(1 << 28) + 1), dict(pt_trace_lds_depth=4), dict(pt_trace_waves_per_cu=2000),
+               dict(trace_reorder_rays=-2)):
         with pytest.raises(mcrt_amd.McrtError) as e:
             s.set_tuning(**kw)
         assert e.value.code == 1
