@@ -266,7 +266,9 @@ MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* scene, mcrt_tuning* tuning)
 
 /* Autotuner (no reference counterpart): times closest-hit over the n rays
  * for trace_waves_per_cu in {8,16,24,32,64,128} x trace_lds_depth in
- * {8,12,16} and keeps the fastest; with `render` != NULL also times one
+ * {8,12,16} and keeps the fastest, then trace_reorder_rays -1 against
+ * min(n, 2^20) (the caller's ray order against origin-cell order) with the
+ * chosen pair, and keeps the faster; with `render` != NULL also times one
  * render per pt_trace_waves_per_cu in {16,32,64} x pt_trace_lds_depth in
  * {8,16}, then per pt_chunk_paths in 2^20..2^24, keeping the fastest.  The
  * chosen tuning is applied to the scene and returned in `best`, the best

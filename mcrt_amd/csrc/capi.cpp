@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -530,6 +531,27 @@ MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcr
             }
         s->trace_waves_per_cu = bw;
         s->trace_lds_depth = bd;
+        // then the ray order: the caller's, or by origin cell from min(n, 2^20) rays
+        const int64_t order[2] = {-1, (int64_t)n < (int64_t(1) << 20) ? (int64_t)n : int64_t(1) << 20};
+        double order_rate[2] = {0.0, 0.0};
+        for (int c = 0; c < 2; c++)
+        {
+            s->trace_reorder_rays = order[c];
+            apply_trace_tuning(s);
+            DevScene view;
+            if ((rc = launch_view(s, res, view, (int64_t)n))) return rc;
+            HIPCHK(launch_trace(view, r, (int64_t)n, d_hits.p, nullptr, TRACE_CLOSEST, nullptr, st));
+            HIPCHK(hipEventRecord(e0, st));
+            for (int k = 0; k < 3; k++)
+                HIPCHK(launch_trace(view, r, (int64_t)n, d_hits.p, nullptr, TRACE_CLOSEST, nullptr, st));
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            order_rate[c] = 3.0 * (double)n / (ms * 1e-3) / 1e6;
+        }
+        s->trace_reorder_rays = order_rate[1] > order_rate[0] ? order[1] : order[0];
+        best_rate = std::max(order_rate[0], order_rate[1]);
         apply_trace_tuning(s);
     }
     // Path tracer: its traversal launches (waves per CU x LDS depth), then the

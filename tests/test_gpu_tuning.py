@@ -116,6 +116,7 @@ def test_autotune_applies_valid_tuning(ctx):
     assert t["trace_waves_per_cu"] in (8, 16, 24, 32, 64, 128)
     assert t["pt_chunk_paths"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
     assert t["pt_trace_waves_per_cu"] in (16, 32, 64) and t["pt_trace_lds_depth"] in (8, 16)
+    assert t["trace_reorder_rays"] in (-1, rays.shape[0])
     assert rate > 0
     assert np.array_equal(s.intersect(rays), before)
     assert np.array_equal(s.render(spp=4, max_depth=4, seed=2).view(np.uint32), film0.view(np.uint32))
