@@ -62,3 +62,34 @@ def test_python_binding_refuses_missing_library(monkeypatch, tmp_path):
     monkeypatch.setattr(api, "LIB_PATH", str(tmp_path / "missing.so"))
     with pytest.raises(api.McrtError):
         api.lib()
+
+
+def test_python_structs_match_the_header(tmp_path):
+    """Every ctypes mirror of a public struct has the C header's size and
+    field offsets (a probe compiled against include/mcrt.h with g++)."""
+    import shutil
+    import subprocess
+    from mcrt_amd import api
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    mirrors = {"mcrt_build_stats": api.BuildStats, "mcrt_build_opts": api.BuildOpts,
+               "mcrt_render_params": api.RenderParams, "mcrt_tuning": api.Tuning}
+    lines = ['#include <cstddef>', '#include <cstdio>', '#include "mcrt.h"', "int main() {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  std::printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  std::printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append('  std::printf("mcrt_hit size %zu\\n", sizeof(mcrt_hit));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "probe.cpp"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["g++", "-std=c++17", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+    assert got[("mcrt_hit", "size")] == 16
