@@ -237,7 +237,10 @@ def c3_leg(ctx, steps):
     log(f"C3 soup SBVH: {st['build_seconds']:.2f} s build, {st['refs']} refs; cache load {st2['build_seconds']:.2f} s")
     out = {"config": {"triangles": int(tris.shape[0]), "rays": scenes.C3_RAYS, "refs": st["refs"],
                       "wide_nodes": st["wide_nodes"], "sbvh_build_s": round(st["build_seconds"], 2),
-                      "any_hit_tfar": scenes.C3_ANY_TFAR},
+                      "any_hit_tfar": scenes.C3_ANY_TFAR,
+                      # queries of at least this many rays run in origin-cell order (-1: never);
+                      # finalize's default for a BVH larger than the last-level cache
+                      "reorder_rays": s.tuning()["trace_reorder_rays"]},
            "bvh_cache": {"build_and_write_s": round(st["build_seconds"], 2),
                          "load_s": round(st2["build_seconds"], 2), "file_mb": round(cache_mb, 1)}}
     n = scenes.C3_RAYS
