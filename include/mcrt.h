@@ -248,7 +248,7 @@ typedef struct mcrt_tuning {
                                    16 or 24 (8); deeper entries go to a per-stream
                                    global spill area                                    */
     int64_t pt_chunk_paths;     /* paths per wavefront chunk of the renders, 1..2^28
-                                   (2^24); path state is ~0.5 KB per path              */
+                                   (2^26); path state is ~0.5 KB per path              */
     int32_t pt_trace_waves_per_cu; /* the same two settings for the traversal
                                       launches of renders, whose ray sets (many
                                       short shadow rays) favour more waves: (32) */
@@ -270,7 +270,7 @@ MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* scene, mcrt_tuning* tuning)
  * min(n, 2^20) (the caller's ray order against origin-cell order) with the
  * chosen pair, and keeps the faster; with `render` != NULL also times one
  * render per pt_trace_waves_per_cu in {16,32,64} x pt_trace_lds_depth in
- * {8,16}, then per pt_chunk_paths in 2^20..2^24, keeping the fastest.  The
+ * {8,16}, then per pt_chunk_paths in 2^22..2^26, keeping the fastest.  The
  * chosen tuning is applied to the scene and returned in `best`, the best
  * closest-hit rate in `best_mrays_per_s` (either may be NULL).  Rays follow
  * `flags` like the queries; synchronous. */

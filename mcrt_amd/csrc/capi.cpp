@@ -593,10 +593,10 @@ MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcr
         s->pt_trace_lds_depth = pd;
         int64_t bc = s->pt_chunk_paths;
         best_ms = 0.0f;
-        for (int lg = 20; lg <= 24; lg++)
+        for (int lg = 22; lg <= 26; lg++)
         {
             const int64_t c = int64_t(1) << lg;
-            if (lg > 20 && (c >> 1) >= total) break;  // larger chunks change nothing
+            if (lg > 22 && (c >> 1) >= total) break;  // larger chunks change nothing
             s->pt_chunk_paths = c;
             float ms = 0.0f;
             if ((rc = time_render(ms))) return rc;

@@ -114,7 +114,7 @@ def test_autotune_applies_valid_tuning(ctx):
     assert t == s.tuning()
     assert t["trace_lds_depth"] in (8, 12, 16)
     assert t["trace_waves_per_cu"] in (8, 16, 24, 32, 64, 128)
-    assert t["pt_chunk_paths"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
+    assert t["pt_chunk_paths"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
     assert t["pt_trace_waves_per_cu"] in (16, 32, 64) and t["pt_trace_lds_depth"] in (8, 16)
     assert t["trace_reorder_rays"] in (-1, rays.shape[0])
     assert rate > 0
