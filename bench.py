@@ -339,8 +339,8 @@ def main():
                     help="only run a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(every GPU box has one GPU).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the path runs on a "
+              "single GPU (how it would shard is described with the design).", file=sys.stderr)
         sys.exit(2)
 
     import torch
