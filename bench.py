@@ -80,6 +80,20 @@ def cpu_baseline(scene_data, rays_np, sample):
             "reference_default_threads": {"cores": int(max(1, cores - 1)), "value": round(sample / dt1 / 1e6, 3)}}
 
 
+def timed_band(render, cores):
+    """Seconds of render(threads) with every core (median of 3, after which
+    the film of the last run is returned) and with all cores but one, the
+    reference demo's own default (demo_pt.cpp:726)."""
+    times = []
+    for _ in range(3):
+        t0 = time.time()
+        film = render(cores)
+        times.append(time.time() - t0)
+    t0 = time.time()
+    render(max(1, cores - 1))
+    return sorted(times)[1], time.time() - t0, film
+
+
 RENDER = dict(width=1024, height=1024, spp=64, max_depth=8, rr_start=3, rr_prob=0.8, seed=1)
 
 
@@ -113,18 +127,9 @@ def render_leg(scene, data, steps, warm, cpu):
         osc = _libs.OracleScene(data)
         cores = _libs.oracle().orc_num_procs()
         rows = (R["height"] // 2 - 8, R["height"] // 2 + 8)
-        times = []
-        for _ in range(3):
-            t0 = time.time()
-            cfilm = osc.render(R["width"], R["height"], R["spp"], R["max_depth"], R["rr_start"], R["rr_prob"],
-                               R["seed"], rows=rows, threads=cores)
-            times.append(time.time() - t0)
-        dt = sorted(times)[1]
-        # the reference demo's own default: all cores but one (demo_pt.cpp:726)
-        t0 = time.time()
-        osc.render(R["width"], R["height"], R["spp"], R["max_depth"], R["rr_start"], R["rr_prob"], R["seed"],
-                   rows=rows, threads=max(1, cores - 1))
-        dt1 = time.time() - t0
+        dt, dt1, cfilm = timed_band(
+            lambda t: osc.render(R["width"], R["height"], R["spp"], R["max_depth"], R["rr_start"], R["rr_prob"],
+                                 R["seed"], rows=rows, threads=t), cores)
         cs = (rows[1] - rows[0]) * R["width"] * R["spp"]
         band = film[rows[0]:rows[1]].cpu().numpy()
         same = bool(np.array_equal(band.view(np.uint32), cfilm[rows[0]:rows[1]].view(np.uint32)))
@@ -177,16 +182,17 @@ def env_leg(ctx, steps, cpu):
         osc.set_envmap(env)
         cores = _libs.oracle().orc_num_procs()
         rows = (E["height"] // 2 - 8, E["height"] // 2 + 8)
-        t0 = time.time()
-        cfilm = osc.render(E["width"], E["height"], E["spp"], E["max_depth"], E["rr_start"], E["rr_prob"],
-                           E["seed"], rows=rows, threads=cores)
-        dt = time.time() - t0
+        dt, dt1, cfilm = timed_band(
+            lambda t: osc.render(E["width"], E["height"], E["spp"], E["max_depth"], E["rr_start"], E["rr_prob"],
+                                 E["seed"], rows=rows, threads=t), cores)
         cs = (rows[1] - rows[0]) * E["width"] * E["spp"]
         band = film[rows[0]:rows[1]].cpu().numpy()
         out["cpu_baseline"] = {"value": round(cs / dt / 1e6, 4), "unit": "Msamples/s", "cores": int(cores),
-                               "kind": "port", "sample": f"rows {rows[0]}..{rows[1]} ({cs} samples)",
+                               "kind": "port", "sample": f"rows {rows[0]}..{rows[1]} ({cs} samples), median of 3",
                                "band_bit_identical_to_gpu":
-                                   bool(np.array_equal(band.view(np.uint32), cfilm[rows[0]:rows[1]].view(np.uint32)))}
+                                   bool(np.array_equal(band.view(np.uint32), cfilm[rows[0]:rows[1]].view(np.uint32))),
+                               "reference_default_threads": {"cores": int(max(1, cores - 1)),
+                                                             "value": round(cs / dt1 / 1e6, 4)}}
         out["gpu_over_cpu"] = round(out["msamples_per_s"] / out["cpu_baseline"]["value"], 1)
     s.close()
     return out
