@@ -126,7 +126,10 @@ MCRT_API int mcrt_material_add(mcrt_scene* scene, int type, const float rgb[3], 
 MCRT_API int mcrt_materials_add(mcrt_scene* scene, size_t n, const int32_t* types, const float* rgb,
                                 const float* glossiness, int32_t* first_id);
 /* Bulk replacement of material colours (albedo / emission) after finalize,
- * for optimisation loops; rgb holds 3 floats per material id. */
+ * for optimisation loops; rgb holds 3 floats per material id.  After
+ * finalize it waits for the device (work already enqueued on any stream,
+ * batch replays included, reads the old colours) and then uploads the new
+ * table synchronously. */
 MCRT_API int mcrt_material_set_rgb(mcrt_scene* scene, const float* rgb, size_t nmat);
 /* Scene::setEnvmap(filename) and setEnvmap(filename, width, height)
  * (scene.h:44-45; EnvmapLoader, envmap.cpp:98-140): a latitude-longitude map
@@ -153,6 +156,17 @@ MCRT_API int mcrt_scene_add_triangles(mcrt_scene* scene, const float* v9, const 
  * normals; out_n (optional) receives the number of faces added. */
 MCRT_API int mcrt_scene_load_obj(mcrt_scene* scene, const char* path, const float* m16,
                                  int32_t mat, size_t* out_n);
+/* Mesh(path) with the transform and genNormals of Scene::loadModel
+ * (src/core/mesh.cpp:24-118, src/core/scene.cpp:368-406), without a scene:
+ * the faces as a triangle soup, for callers that give each triangle its own
+ * material through mcrt_scene_add_triangles (the reference's OBJ path has
+ * one material per file).  v9 and vn9 receive n x 9 floats (corners a, b,
+ * c; vertex normals na, nb, nc); `cap` is the number of triangles they
+ * hold.  With v9 = NULL only *out_n (the face count) is written.  A `v` or
+ * `f` token followed by something other than numbers is MCRT_ERR_IO (the
+ * reference's loop never ends on it).  No device or context is needed. */
+MCRT_API int mcrt_obj_read(const char* path, const float* m16, float* v9, float* vn9, size_t cap,
+                           size_t* out_n);
 /* Scene::setCamera(origin, target, W, H, fov_y), up = (0, 1, 0). */
 MCRT_API int mcrt_scene_set_camera(mcrt_scene* scene, const float o[3], const float target[3],
                                    int32_t width, int32_t height, float fov_y);

@@ -6,7 +6,8 @@ ABI in include/mcrt.h); this package holds its ctypes binding, the scene
 generators used by the benchmarks and tests, and the build recipe.
 """
 from .api import (DEVICE_PTRS, DIFF, GLSY, LGHT, BuildStats, Context, McrtError, RenderParams, Scene, lib,
-                  split_hits, tonemap_gamma, write_hdr, write_ppm, Batch, render_batch)
+                  split_hits, tonemap_gamma, write_hdr, write_ppm, Batch, render_batch, obj_read)
 
 __all__ = ["Context", "Scene", "McrtError", "BuildStats", "RenderParams", "lib", "split_hits", "LGHT", "DIFF",
-           "GLSY", "DEVICE_PTRS", "write_hdr", "write_ppm", "tonemap_gamma", "Batch", "render_batch"]
+           "GLSY", "DEVICE_PTRS", "write_hdr", "write_ppm", "tonemap_gamma", "Batch", "render_batch",
+           "obj_read"]

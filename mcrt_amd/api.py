@@ -83,6 +83,7 @@ def lib():
         "mcrt_scene_set_envmap_rgb": (I, [P, P, ctypes.c_int32, ctypes.c_int32]),
         "mcrt_scene_add_triangles": (I, [P, P, P, P, Z]),
         "mcrt_scene_load_obj": (I, [P, ctypes.c_char_p, P, ctypes.c_int32, ctypes.POINTER(Z)]),
+        "mcrt_obj_read": (I, [ctypes.c_char_p, P, P, P, Z, ctypes.POINTER(Z)]),
         "mcrt_scene_set_camera": (I, [P, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float]),
         "mcrt_scene_finalize": (I, [P, ctypes.POINTER(BuildOpts), ctypes.POINTER(BuildStats)]),
         "mcrt_scene_num_triangles": (I, [P, ctypes.POINTER(Z)]),
@@ -450,6 +451,20 @@ def tonemap_gamma(film, gamma=2.2):
     f = _host_film(film).copy()
     _check(lib().mcrt_film_tonemap_gamma(f.ctypes.data, f.size, gamma))
     return f
+
+
+def obj_read(path, transform=None):
+    """Mesh(path) with Scene::loadModel's transform (column-major 4x4 as a
+    numpy 4x4 matrix, applied to column vectors) and vertex normals, as a
+    triangle soup: (v9, vn9), each n x 9 float32.  No GPU needed."""
+    m = None if transform is None else np.ascontiguousarray(np.asarray(transform, np.float32).T.reshape(16))
+    mp = None if m is None else m.ctypes.data
+    n = ctypes.c_size_t()
+    _check(lib().mcrt_obj_read(os.fsencode(path), mp, None, None, 0, ctypes.byref(n)))
+    v9 = np.empty((n.value, 9), np.float32)
+    vn9 = np.empty((n.value, 9), np.float32)
+    _check(lib().mcrt_obj_read(os.fsencode(path), mp, v9.ctypes.data, vn9.ctypes.data, n.value, ctypes.byref(n)))
+    return v9, vn9
 
 
 def split_hits(hits):

@@ -205,6 +205,9 @@ MCRT_API int mcrt_material_set_rgb(mcrt_scene* s, const float* rgb, size_t nmat)
     if (s->finalized)
     {
         HIPCHK(hipSetDevice(s->ctx->device));
+        // renders, adjoints and batch replays enqueued on any stream read the
+        // live table: let them finish before it is overwritten
+        HIPCHK(hipDeviceSynchronize());
         int rc = upload_materials(s);
         if (rc) return rc;
     }
@@ -270,20 +273,25 @@ MCRT_API int mcrt_scene_load_obj(mcrt_scene* s, const char* path, const float* m
     if (rc) return rc;
     const size_t nf = m.f.size() / 3;
     std::vector<float> v9(9 * nf), vn9(9 * nf);
-    for (size_t i = 0; i < nf; i++)
-        for (int c = 0; c < 3; c++)
-        {
-            int vi = m.f[3 * i + c];
-            for (int k = 0; k < 3; k++)
-            {
-                v9[9 * i + 3 * c + k] = m.v[vi][k];
-                vn9[9 * i + 3 * c + k] = m.vn[vi][k];
-            }
-        }
+    obj_soup(m, v9.data(), vn9.data());
     std::vector<int32_t> mats(nf, mat);
     rc = mcrt_scene_add_triangles(s, v9.data(), vn9.data(), mats.data(), nf);
     if (rc) return rc;
     if (out_n) *out_n = nf;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_obj_read(const char* path, const float* m16, float* v9, float* vn9, size_t cap, size_t* out_n)
+{
+    if (!path || !out_n || (v9 && !vn9)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_obj_read: NULL argument");
+    ObjMesh m;
+    const int rc = load_obj(path, m16, m);
+    if (rc) return rc;
+    const size_t nf = m.f.size() / 3;
+    *out_n = nf;
+    if (!v9) return MCRT_OK;
+    if (nf > cap) return fail(MCRT_ERR_INVALID_ARG, "mcrt_obj_read: %s has %zu faces, buffers hold %zu", path, nf, cap);
+    obj_soup(m, v9, vn9);
     return MCRT_OK;
 }
 
@@ -339,7 +347,8 @@ static int check_render(const mcrt_scene* s, const mcrt_render_params* p, const 
 {
     if (!s->finalized) return fail(MCRT_ERR_STATE, "%s: scene not finalized", who);
     if (!s->has_cam) return fail(MCRT_ERR_STATE, "%s: no camera (mcrt_scene_set_camera)", who);
-    if (p->spp < 1) return fail(MCRT_ERR_INVALID_ARG, "%s: spp must be >= 1", who);
+    // one pixel's samples stay in one wavefront chunk (at most 2^28 paths)
+    if (p->spp < 1 || p->spp > (1 << 28)) return fail(MCRT_ERR_INVALID_ARG, "%s: spp must be in [1, 2^28]", who);
     if (p->max_depth < 0 || p->max_depth > 250) return fail(MCRT_ERR_INVALID_ARG, "%s: max_depth in [0, 250]", who);
     if (!(p->rr_prob > 0.0f && p->rr_prob <= 1.0f)) return fail(MCRT_ERR_INVALID_ARG, "%s: rr_prob in (0, 1]", who);
     if (p->sample_offset < 0) return fail(MCRT_ERR_INVALID_ARG, "%s: sample_offset must be >= 0", who);
@@ -469,10 +478,36 @@ MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* s, mcrt_tuning* t)
     return MCRT_OK;
 }
 
+static int autotune(mcrt_scene* s, const float* rays, size_t n, const mcrt_render_params* render, int flags,
+                    void* stream, mcrt_tuning* best, double* best_mrays_per_s);
+
 MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcrt_render_params* render, int flags,
                            void* stream, mcrt_tuning* best, double* best_mrays_per_s)
 {
-    if (!s || (n && !rays)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_autotune: NULL argument");
+    if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_autotune: NULL argument");
+    mcrt_tuning before;
+    mcrt_scene_get_tuning(s, &before);
+    const int rc = autotune(s, rays, n, render, flags, stream, best, best_mrays_per_s);
+    if (rc)
+    {
+        // a failed sweep leaves the scene as it found it
+        const std::string err = mcrt_last_error();
+        s->trace_waves_per_cu = before.trace_waves_per_cu;
+        s->trace_lds_depth = before.trace_lds_depth;
+        s->pt_chunk_paths = before.pt_chunk_paths;
+        s->pt_trace_waves_per_cu = before.pt_trace_waves_per_cu;
+        s->pt_trace_lds_depth = before.pt_trace_lds_depth;
+        s->trace_reorder_rays = before.trace_reorder_rays;
+        if (s->finalized) apply_trace_tuning(s);
+        fail(rc, "%s", err.c_str());
+    }
+    return rc;
+}
+
+static int autotune(mcrt_scene* s, const float* rays, size_t n, const mcrt_render_params* render, int flags,
+                    void* stream, mcrt_tuning* best, double* best_mrays_per_s)
+{
+    if (n && !rays) return fail(MCRT_ERR_INVALID_ARG, "mcrt_autotune: NULL argument");
     if (!s->finalized) return fail(MCRT_ERR_STATE, "mcrt_autotune: scene not finalized");
     if (n == 0 && !render) return fail(MCRT_ERR_INVALID_ARG, "mcrt_autotune: nothing to time (no rays, no render)");
     int rc;

@@ -56,6 +56,8 @@ struct ObjMesh
     std::vector<int> f;  // 3 per face, 0-based
 };
 int load_obj(const char* path, const float* m16, ObjMesh& out);
+// The faces of `m` as a triangle soup: v9 / vn9 = nf x (a, b, c) positions / vertex normals.
+void obj_soup(const ObjMesh& m, float* v9, float* vn9);
 }  // namespace mcrt
 
 namespace mcrt

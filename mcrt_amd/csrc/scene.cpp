@@ -26,6 +26,11 @@ namespace mcrt
 // (m0*x + m1*y) + (m2*z + m3*w) (type_mat4x4.inl:532-543), then
 // Mesh::genNormals (mesh.cpp:66-92): vertex normal = normalize of the sum
 // of the unnormalised face normals (vb-va) x (vc-va) in face order.
+// A `v` or `f` token followed by something that is not a number before the
+// end of the file (`f 1/1/1 ...`, a `v` word inside a comment) leaves the
+// reference's stream failed but not at its end, so its loop never ends; here
+// it is an MCRT_ERR_IO naming the line.  A record cut off by the end of the
+// file is dropped, as the reference's loop ends there too.
 int load_obj(const char* path, const float* m16, ObjMesh& m)
 {
     std::ifstream ifs(path, std::ios::binary);
@@ -33,13 +38,22 @@ int load_obj(const char* path, const float* m16, ObjMesh& m)
     std::stringstream reader;
     reader << ifs.rdbuf();
     std::string token;
-    while (!reader.eof())
+    auto bad_record = [&](const char* what) {
+        reader.clear();
+        const std::string& text = reader.str();
+        const std::streamoff pos = reader.tellg();
+        const size_t end = pos < 0 ? text.size() : std::min(text.size(), (size_t)pos);
+        const long line = 1 + (long)std::count(text.begin(), text.begin() + (std::ptrdiff_t)end, '\n');
+        return fail(MCRT_ERR_IO, "%s:%ld: unsupported %s record (only 'v x y z' and 'f a b c' are read)", path,
+                    line, what);
+    };
+    while (reader >> token)
     {
-        reader >> token;
         if (token == "v")
         {
             float a, b, c;
             if (reader >> a >> b >> c) m.v.push_back(f3(a, b, c));
+            else if (!reader.eof()) return bad_record("vertex");
         }
         else if (token == "f")
         {
@@ -50,6 +64,7 @@ int load_obj(const char* path, const float* m16, ObjMesh& m)
                 m.f.push_back((int)(b - 1));
                 m.f.push_back((int)(c - 1));
             }
+            else if (!reader.eof()) return bad_record("face");
         }
     }
     for (int idx : m.f)
@@ -75,6 +90,23 @@ int load_obj(const char* path, const float* m16, ObjMesh& m)
     }
     for (F3& n : m.vn) n = normalize3(n);
     return MCRT_OK;
+}
+
+// Scene::loadModel's per-face TriangleObject(a, b, c, na, nb, nc)
+// (scene.cpp:384-406): corners and vertex normals by the face's indices.
+void obj_soup(const ObjMesh& m, float* v9, float* vn9)
+{
+    const size_t nf = m.f.size() / 3;
+    for (size_t i = 0; i < nf; i++)
+        for (int c = 0; c < 3; c++)
+        {
+            const int vi = m.f[3 * i + c];
+            for (int k = 0; k < 3; k++)
+            {
+                v9[9 * i + 3 * c + k] = m.v[vi][k];
+                vn9[9 * i + 3 * c + k] = m.vn[vi][k];
+            }
+        }
 }
 
 // Camera::Camera (camera.cpp:9-35), up = (0,1,0) (scene.cpp:231).

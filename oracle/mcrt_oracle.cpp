@@ -640,20 +640,22 @@ struct Mesh
     std::vector<V3> vn;
 };
 
-static bool load_obj(const char* path, Mesh& m)
+// A `v`/`f` token followed by a non-number before the end of the file (the
+// reference's loop never ends on it) is reported as failure (-2).
+static int load_obj(const char* path, Mesh& m)
 {
     std::ifstream ifs(path, std::ios::binary);
-    if (!ifs.good()) return false;
+    if (!ifs.good()) return -1;
     std::stringstream reader;
     reader << ifs.rdbuf();
     std::string token;
-    while (!reader.eof())
+    while (reader >> token)
     {
-        reader >> token;
         if (token == "v")
         {
             float a, b, c;
             if (reader >> a >> b >> c) m.v.push_back(v3(a, b, c));
+            else if (!reader.eof()) return -2;
         }
         else if (token == "f")
         {
@@ -662,9 +664,33 @@ static bool load_obj(const char* path, Mesh& m)
             {
                 m.f.push_back((int)(a - 1)); m.f.push_back((int)(b - 1)); m.f.push_back((int)(c - 1));
             }
+            else if (!reader.eof()) return -2;
         }
     }
-    return true;
+    for (int idx : m.f)
+        if (idx < 0 || (size_t)idx >= m.v.size()) return -2;
+    return 0;
+}
+
+// Scene::loadModel (scene.cpp:368-381): v' = transform * vec4(v, 1) with
+// GLM's tmat4x4 * tvec4 (type_mat4x4.inl:532-543): per output row r,
+// (m[0][r]*x + m[1][r]*y) + (m[2][r]*z + m[3][r]*w); m16 is column-major,
+// m16[4*c + r] = m[c][r].
+static void transform_vertices(Mesh& m, const float* m16)
+{
+    for (auto& p : m.v)
+    {
+        const float x = p.x, y = p.y, z = p.z, w = 1.0f;
+        float q[3];
+        for (int r = 0; r < 3; r++)
+        {
+            const float mul0 = m16[0 + r] * x, mul1 = m16[4 + r] * y;
+            const float mul2 = m16[8 + r] * z, mul3 = m16[12 + r] * w;
+            const float add0 = mul0 + mul1, add1 = mul2 + mul3;
+            q[r] = add0 + add1;
+        }
+        p = v3(q[0], q[1], q[2]);
+    }
 }
 
 // Mesh::genNormals (mesh.cpp:66-92): vertex normal = normalize(sum of
@@ -780,16 +806,21 @@ ORC_API void orc_brute_intersect(const float* tris9, int ntri, const float* rays
     }
 }
 
-// OBJ ingest (mesh.cpp:24-118 + scene.cpp:368-408 with an identity
-// transform): returns the face count, or -1 if the file cannot be opened.
-// With out buffers NULL only counts.  tris: nf x 9, vn: nf x 9 (vertex
-// normals per corner).
-ORC_API long long orc_load_obj(const char* path, float* tris, float* vn)
+// OBJ ingest (mesh.cpp:24-118 + scene.cpp:368-408): returns the face count,
+// -1 if the file cannot be opened, -2 for a record that is not `v x y z` /
+// `f a b c` or a face index out of range.  m16 = column-major transform or
+// NULL (identity: the reference's transform leaves the bits unchanged; its
+// second genNormals gives the same normals as the first).  With out
+// buffers NULL only counts.  tris: nf x 9, vn: nf x 9 (vertex normals per
+// corner).
+ORC_API long long orc_load_obj(const char* path, const float* m16, float* tris, float* vn)
 {
     Mesh m;
-    if (!load_obj(path, m)) return -1;
+    const int rc = load_obj(path, m);
+    if (rc) return rc;
     long long nf = (long long)m.f.size() / 3;
     if (!tris) return nf;
+    if (m16) transform_vertices(m, m16);
     gen_normals(m);
     for (long long i = 0; i < nf; i++)
         for (int k = 0; k < 3; k++)

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include <glm/glm.hpp>
+#include <glm/gtc/matrix_transform.hpp>
 
 #include "aabb.h"
 #include "boundedtriangle.h"
@@ -149,14 +150,19 @@ REF_API void ref_bvh_intersect(void* h, const float* rays, long long n, int* pri
 }
 
 // The plug-in boundary itself: SceneBVH over TriangleObjects (scenebvh.cpp:340-445).
-// rays n x 6 (o, d); out: prim, t/u/v, shading normal n x 3; occluded with tfar.
-REF_API void ref_scenebvh_query(const float* tris9, int ntri, const float* rays6, long long n,
+// Triangles tris9 with vertex normals vn9 (n x 9) or, for NULL, the face
+// normal (TriangleObject(a, b, c)); rays n x 6 (o, d); out: prim, t/u/v,
+// shading normal n x 3; occluded with tfar.
+REF_API void ref_scenebvh_query(const float* tris9, const float* vn9, int ntri, const float* rays6, long long n,
                                 float tfar, int* prim, float* tuv, float* nrm, uint8_t* occ)
 {
     std::vector<TriangleObject> tris;
     for (int i = 0; i < ntri; i++)
     {
-        TriangleObject t(V(tris9 + 9 * i), V(tris9 + 9 * i + 3), V(tris9 + 9 * i + 6));
+        const float* p = tris9 + 9 * i;
+        TriangleObject t = vn9 ? TriangleObject(V(p), V(p + 3), V(p + 6), V(vn9 + 9 * i), V(vn9 + 9 * i + 3),
+                                                V(vn9 + 9 * i + 6))
+                               : TriangleObject(V(p), V(p + 3), V(p + 6));
         t.setMaterialID(0);
         tris.push_back(t);
     }
@@ -268,6 +274,38 @@ REF_API long long ref_load_obj(const char* path, const float* m16, float* tris9,
         }
     }
     return nf;
+}
+
+// The two loadModel transforms of the demo (demo_pt.cpp:42-61), made by
+// GLM: out32[0..15] = translate(mat4(1), vec3(0.5, 0, 0)) and
+// out32[16..31] = scale(mat4(1), vec3(0.015)) * rotate(mat4(1),
+// radians(300), vec3(0, 1, 0)), each column-major (out[4c + r] = m[c][r]).
+REF_API void ref_demo_transforms(float* out32)
+{
+    const glm::mat4 shift = glm::translate(glm::mat4(1.0f), glm::vec3(0.5f, 0.0f, 0.0f));
+    const glm::mat4 rotation = glm::rotate(glm::mat4(1.0f), glm::radians(300.0f), glm::vec3(0.0f, 1.0f, 0.0f));
+    const glm::mat4 scaling = glm::scale(glm::mat4(1.0f), glm::vec3(0.015f));
+    const glm::mat4 sr = scaling * rotation;
+    for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++)
+        {
+            out32[4 * c + r] = shift[c][r];
+            out32[16 + 4 * c + r] = sr[c][r];
+        }
+}
+
+// TriangleObject(a, b, c, na, nb, nc).normal(u, v) (triangle.cpp:90-93) for
+// n triangles (tris9, vn9) at barycentrics uv2: out n x 3.
+REF_API void ref_shading_normal(const float* tris9, const float* vn9, const float* uv2, long long n, float* out3)
+{
+    for (long long i = 0; i < n; i++)
+    {
+        const float* t = tris9 + 9 * i;
+        const float* q = vn9 + 9 * i;
+        TriangleObject tri(V(t), V(t + 3), V(t + 6), V(q), V(q + 3), V(q + 6));
+        const vec3 r = tri.normal(uv2[2 * i], uv2[2 * i + 1]);
+        out3[3 * i] = r.x; out3[3 * i + 1] = r.y; out3[3 * i + 2] = r.z;
+    }
 }
 
 // Camera::makeRay at (mx, my) = pxy / (w, h); rays6 n x 6.

@@ -37,7 +37,7 @@ def oracle():
         "orc_bvh_digest": (U64, [P, P]),
         "orc_bvh_intersect": (None, [P, P, LL, P, P, P, I]),
         "orc_brute_intersect": (None, [P, I, P, LL, P, P, I]),
-        "orc_load_obj": (LL, [ctypes.c_char_p, P, P]),
+        "orc_load_obj": (LL, [ctypes.c_char_p, P, P, P]),
         "orc_num_procs": (I, []),
         "orc_scene_create": (P, [P, P, P, I, P, P, P, I]),
         "orc_scene_free": (None, [P]),
@@ -58,6 +58,10 @@ def oracle():
         "orc_env_query": (None, [P, P, LL, I, P]),
         "orc_env_sample": (None, [P, P, LL, P]),
         "orc_env_tables": (ctypes.c_float, [P, P, P, P]),
+        "orc_lambert_eval": (None, [P, LL, P, I, P]),
+        "orc_shading_normal": (None, [P, P, P, LL, P]),
+        "orc_tent": (None, [P, LL, P]),
+        "orc_pixel_jitter": (None, [U64, P, P, LL, P]),
     })
 
 
@@ -67,12 +71,14 @@ def reference():
         "ref_bvh_free": (None, [P]),
         "ref_bvh_digest": (U64, [P, P]),
         "ref_bvh_intersect": (None, [P, P, LL, P, P]),
-        "ref_scenebvh_query": (None, [P, I, P, LL, F, P, P, P, P]),
+        "ref_scenebvh_query": (None, [P, P, I, P, LL, F, P, P, P, P]),
         "ref_debug_mesh": (I, [P]),
         "ref_ray_box": (I, [P, P]),
         "ref_ray_tri": (I, [P, P, P]),
         "ref_clip_bound_tight": (None, [P, P, F, F, I, P]),
         "ref_load_obj": (LL, [ctypes.c_char_p, P, P, P]),
+        "ref_demo_transforms": (None, [P]),
+        "ref_shading_normal": (None, [P, P, P, LL, P]),
         "ref_camera_rays": (None, [P, P, I, I, F, P, LL, P]),
         "ref_bsdf": (None, [I, P, P, LL, P]),
         "ref_discrete": (None, [P, I, P, LL, P, P]),

@@ -241,8 +241,110 @@ def film_fixture(R):
     print("film", {k: v.size for k, v in out.items()})
 
 
+REF_INPUT = "/root/reference/input"
+OBJ_MODELS = ("bunny_1k", "unitsph")
+
+
+def last_face_mask(tris):
+    """Triangles that share no vertex position with the last face.  When a
+    file ends with a newline after its last face, the reference's Mesh adds
+    a phantom face (a failed extraction at end of file; the -O3 build repeats
+    the last face) whose normal is counted again at those vertices."""
+    last = set(map(tuple, tris[-1].reshape(3, 3).tolist()))
+    return np.array([not any(tuple(p) in last for p in t.reshape(3, 3).tolist()) for t in tris])
+
+
+def obj_model_path(name, tmpdir):
+    """Path of a committed reference OBJ (unitsph is stored gzip-compressed)."""
+    import gzip
+    import shutil
+    if os.path.exists(os.path.join(HERE, name + ".obj")):
+        return os.path.join(HERE, name + ".obj")
+    out = os.path.join(tmpdir, name + ".obj")
+    with gzip.open(os.path.join(HERE, name + ".obj.gz"), "rb") as src, open(out, "wb") as dst:
+        shutil.copyfileobj(src, dst)
+    return out
+
+
+def obj_models_fixture(R):
+    """Scene::loadModel of the reference's own input/bunny_1k.obj and
+    input/unitsph.obj (copied here: bunny as is, unitsph gzip-compressed)
+    under no transform, an identity matrix and the demo's two transforms
+    (demo_pt.cpp:42-61: translate, scale x rotate, made by GLM): face count,
+    SHA-256 of the corners and of the vertex normals (normals of triangles
+    touching the last face's vertices zeroed, see last_face_mask), and every
+    16th face in full."""
+    import gzip
+    import shutil
+    import tempfile
+    shutil.copyfile(os.path.join(REF_INPUT, "bunny_1k.obj"), os.path.join(HERE, "bunny_1k.obj"))
+    with open(os.path.join(REF_INPUT, "unitsph.obj"), "rb") as src, \
+            gzip.GzipFile(os.path.join(HERE, "unitsph.obj.gz"), "wb", mtime=0) as dst:
+        shutil.copyfileobj(src, dst)
+    m32 = np.zeros(32, np.float32)
+    R.ref_demo_transforms(ptr(m32))
+    mats = {"none": None, "identity": np.eye(4, dtype=np.float32).T.reshape(16).copy(),
+            "translate": m32[:16].copy(), "scale_rotate": m32[16:].copy()}
+    out = {"translate": mats["translate"], "scale_rotate": mats["scale_rotate"]}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name in OBJ_MODELS:
+            path = obj_model_path(name, tmp)
+            for mk, m in mats.items():
+                mp = None if m is None else ptr(m)
+                nf = R.ref_load_obj(path.encode(), mp, None, None)
+                t = np.zeros((nf, 9), np.float32)
+                vn = np.zeros((nf, 9), np.float32)
+                R.ref_load_obj(path.encode(), mp, ptr(t), ptr(vn))
+                # drop a phantom repeat of the last face
+                if nf >= 2 and np.array_equal(t[-1], t[-2]) and np.array_equal(vn[-1], vn[-2]):
+                    t, vn, nf = t[:-1], vn[:-1], nf - 1
+                keep = last_face_mask(t)
+                vm = np.where(keep[:, None], vn, np.float32(0))
+                key = f"{name}_{mk}"
+                out[key + "_n"] = np.int64(nf)
+                out[key + "_tris_sha"] = sha(t)
+                out[key + "_vn_sha"] = sha(vm)
+                out[key + "_tris16"] = t[::16]
+                out[key + "_vn16"] = vm[::16]
+                print(key, nf, "faces,", int(keep.sum()), "normals compared")
+    np.savez_compressed(os.path.join(HERE, "obj_models.npz"), **out)
+
+
+def shading_normal_inputs():
+    """bunny_1k's triangles and vertex normals (through the oracle's OBJ
+    loader, itself pinned to the reference's) plus one random barycentric
+    pair per triangle, edges and corners included."""
+    import tempfile
+    orc = _libs.oracle()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = obj_model_path("bunny_1k", tmp)
+        n = orc.orc_load_obj(path.encode(), None, None, None)
+        t = np.zeros((n, 9), np.float32)
+        vn = np.zeros((n, 9), np.float32)
+        orc.orc_load_obj(path.encode(), None, ptr(t), ptr(vn))
+    rng = np.random.default_rng(31)
+    u = rng.uniform(0, 1, n)
+    v = rng.uniform(0, 1, n) * (1 - u)
+    uv = np.stack([u, v], 1).astype(np.float32)
+    uv[:4] = [[0, 0], [1, 0], [0, 1], [0.5, 0.5]]
+    return t, vn, uv
+
+
+def shading_normal_fixture(R):
+    """TriangleObject(a, b, c, na, nb, nc).normal(u, v) (triangle.cpp:90-93)."""
+    t, vn, uv = shading_normal_inputs()
+    out = np.zeros((t.shape[0], 3), np.float32)
+    R.ref_shading_normal(ptr(t), ptr(vn), ptr(uv), t.shape[0], ptr(out))
+    np.savez_compressed(os.path.join(HERE, "shading_normal.npz"), uv=uv, normal=out, input_sha=sha(t, vn, uv))
+    print("shading normal", t.shape[0])
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["film"]:
+    if sys.argv[1:] == ["obj_models"]:
+        obj_models_fixture(_libs.reference())
+    elif sys.argv[1:] == ["shading_normal"]:
+        shading_normal_fixture(_libs.reference())
+    elif sys.argv[1:] == ["film"]:
         film_fixture(_libs.reference())
     elif sys.argv[1:] == ["camera"]:
         camera_fixture(_libs.reference())
@@ -252,3 +354,5 @@ if __name__ == "__main__":
         main()
         film_fixture(_libs.reference())
         envmap_fixture(_libs.reference())
+        obj_models_fixture(_libs.reference())
+        shading_normal_fixture(_libs.reference())

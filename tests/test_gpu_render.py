@@ -145,10 +145,10 @@ def test_obj_scene_matches_oracle(ctx, tmp_path):
     L = _libs.oracle()
     parts = []
     for p in (light, geo):
-        n = L.orc_load_obj(p.encode(), None, None)
+        n = L.orc_load_obj(p.encode(), None, None, None)
         t = np.zeros((n, 9), np.float32)
         vn = np.zeros((n, 9), np.float32)
-        L.orc_load_obj(p.encode(), _libs.ptr(t), _libs.ptr(vn))
+        L.orc_load_obj(p.encode(), None, _libs.ptr(t), _libs.ptr(vn))
         parts.append((t, vn))
     t = np.concatenate([parts[0][0], parts[1][0]])
     vn = np.concatenate([parts[0][1], parts[1][1]])

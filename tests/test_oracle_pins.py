@@ -188,24 +188,68 @@ def test_traversal_vs_brute_force(orc):
     assert np.all(ot[diff_id, 0] == bt[diff_id, 0])
 
 
-def test_scenebvh_boundary_semantics(orc, ref):
-    """SceneBVH::intersect uses tmin 1e-4 / tmax inf and occluded uses
-    tmax = tfar - 1e-4 (scenebvh.cpp:353-396)."""
-    tris = _libs.debug_mesh()
-    rays = _libs.random_rays(tris, 2048, seed=21)
+def _oracle_normals(orc, tris, vn, prim, tuv):
+    """The oracle's shading normal at each hit (zeros for misses)."""
+    hit = prim >= 0
+    idx = prim[hit]
+    t9 = np.ascontiguousarray(tris[idx])
+    n9 = None if vn is None else np.ascontiguousarray(vn[idx])
+    uv = np.ascontiguousarray(tuv[hit, 1:3])
+    out = np.zeros((idx.shape[0], 3), np.float32)
+    orc.orc_shading_normal(ptr(t9), None if n9 is None else ptr(n9), ptr(uv), idx.shape[0], ptr(out))
+    return hit, out
+
+
+@pytest.mark.parametrize("mesh", ["debug_mesh", "bunny_vertex_normals"])
+def test_scenebvh_boundary_semantics(orc, ref, mesh):
+    """SceneBVH::intersect uses tmin 1e-4 / tmax inf and adds the shading
+    normal TriangleObject::normal(u, v) of the hit; occluded uses
+    tmax = tfar - 1e-4 (scenebvh.cpp:353-396).  The DebugMesh has face
+    normals; the bunny (input/bunny_1k.obj) has the OBJ path's vertex
+    normals, so the barycentric interpolation itself is compared."""
+    if mesh == "debug_mesh":
+        tris, vn = _libs.debug_mesh(), None
+        rays = _libs.random_rays(tris, 2048, seed=21)
+    else:
+        import make_golden
+        tris, vn, _ = make_golden.shading_normal_inputs()
+        rays = _libs.random_rays(tris, 2048, seed=22)
+    n = rays.shape[0]
     r6 = np.ascontiguousarray(np.concatenate([rays[:, :3], rays[:, 4:7]], 1))
-    prim = np.zeros(2048, np.int32)
-    tuv = np.zeros((2048, 3), np.float32)
-    nrm = np.zeros((2048, 3), np.float32)
-    occ = np.zeros(2048, np.uint8)
-    ref.ref_scenebvh_query(ptr(tris), tris.shape[0], ptr(r6), 2048, 0.5, ptr(prim), ptr(tuv), ptr(nrm), ptr(occ))
+    prim = np.zeros(n, np.int32)
+    tuv = np.zeros((n, 3), np.float32)
+    nrm = np.zeros((n, 3), np.float32)
+    occ = np.zeros(n, np.uint8)
+    ref.ref_scenebvh_query(ptr(tris), None if vn is None else ptr(vn), tris.shape[0], ptr(r6), n, 0.5, ptr(prim),
+                           ptr(tuv), ptr(nrm), ptr(occ))
     ob = _libs.OracleBVH(tris)
     p, t = ob.intersect(rays)
     assert np.array_equal(p, prim)
+    assert np.array_equal(bits(t), bits(tuv))
+    hit, on = _oracle_normals(orc, tris, vn, p, t)
+    assert hit.sum() > 300
+    assert np.array_equal(bits(on), bits(nrm[hit]))
     r2 = rays.copy()
     r2[:, 7] = np.float32(0.5) - np.float32(1e-4)
     p2, _ = ob.intersect(r2)
     assert np.array_equal((p2 >= 0).astype(np.uint8), occ)
+
+
+def test_shading_normal_vs_reference_fixture(orc):
+    """TriangleObject(a, b, c, na, nb, nc).normal(u, v) (triangle.cpp:90-93)
+    on the bunny's vertex normals at random barycentrics, corners and an
+    edge midpoint, bit for bit."""
+    import make_golden
+    g = gold("shading_normal.npz")
+    t, vn, uv = make_golden.shading_normal_inputs()
+    assert make_golden.sha(t, vn, uv) == str(g["input_sha"])
+    out = np.zeros((t.shape[0], 3), np.float32)
+    orc.orc_shading_normal(ptr(t), ptr(vn), ptr(uv), t.shape[0], ptr(out))
+    assert np.array_equal(bits(out), bits(g["normal"]))
+    # corners give the (unit) vertex normals themselves
+    assert np.allclose(out[0], vn[0, 0:3], atol=1e-7) and np.allclose(out[1], vn[1, 3:6], atol=1e-7)
+    assert np.allclose(out[2], vn[2, 6:9], atol=1e-7)
+    assert np.allclose(np.linalg.norm(out, axis=1), 1.0, atol=1e-6)
 
 
 # ---------------------------------------------------------- scene / OBJ --
@@ -217,10 +261,10 @@ def test_obj_loader_vs_reference_fixture(orc):
     The oracle keeps the real faces only."""
     g = gold("obj_cornell_ico2.npz")
     path = os.path.join(GOLD, "cornell_ico2.obj")
-    n = orc.orc_load_obj(path.encode(), None, None)
+    n = orc.orc_load_obj(path.encode(), None, None, None)
     t = np.zeros((n, 9), np.float32)
     vn = np.zeros((n, 9), np.float32)
-    orc.orc_load_obj(path.encode(), ptr(t), ptr(vn))
+    orc.orc_load_obj(path.encode(), None, ptr(t), ptr(vn))
     rt, rn = g["tris"], g["normals"]
     assert rt.shape[0] in (n, n + 1)
     assert np.array_equal(bits(t), bits(rt[:n]))
@@ -228,7 +272,7 @@ def test_obj_loader_vs_reference_fixture(orc):
     keep = np.array([not any(tuple(p) in last for p in tri.reshape(3, 3)) for tri in t])
     assert keep.sum() > n // 2
     assert np.array_equal(bits(vn[keep]), bits(rn[:n][keep]))
-    assert orc.orc_load_obj(b"/nonexistent.obj", None, None) == -1
+    assert orc.orc_load_obj(b"/nonexistent.obj", None, None, None) == -1
 
 
 def test_camera_vs_reference_fixture(orc):
@@ -265,6 +309,80 @@ def test_bsdf_sampling_vs_reference_fixture(orc):
             assert np.max(np.abs(out[:, :3] - ref_out[:, :3])) < tol, (typ, prec)
             w = ref_out[:, 3] / g["rgb"][0]
             assert np.max(np.abs(out[:, 3] - w) / np.maximum(1, np.abs(w))) < 20 * tol, (typ, prec)
+
+
+def test_lambert_pdf_and_f_vs_reference_fixture(orc):
+    """LambertianBSDF::calculatePdfW and evaluate (bsdf.cpp:100-124), the
+    pdfW and f of the integrator's MIS weights, at the reference's own
+    sampled directions: bit for bit in fp32; fp64 within rounding."""
+    g = gold("bsdf.npz")
+    inp, ref1 = g["inputs"], g["type1"]
+    n = inp.shape[0]
+    in9 = np.ascontiguousarray(np.concatenate([inp[:, :6], ref1[:, :3]], 1), np.float32)
+    rgb = np.ascontiguousarray(g["rgb"], np.float32)
+    out = np.zeros((n, 4), np.float32)
+    orc.orc_lambert_eval(ptr(in9), n, ptr(rgb), 0, ptr(out))
+    assert np.array_equal(bits(out[:, 0]), bits(ref1[:, 6]))
+    assert np.array_equal(bits(out[:, 1:]), bits(ref1[:, 7:10]))
+    out64 = np.zeros((n, 4), np.float32)
+    orc.orc_lambert_eval(ptr(in9), n, ptr(rgb), 1, ptr(out64))
+    assert np.allclose(out64, out, rtol=1e-6, atol=1e-12)
+    # closed forms: pdf = max(cos, 1e-6) / pi, integrating to 1 over the
+    # hemisphere; f = albedo / pi
+    cos = (in9[:, 6:9].astype(np.float64) * in9[:, 3:6]).sum(1)
+    assert np.allclose(out[:, 0], np.maximum(cos, 1e-6) / np.pi, rtol=1e-6)
+    assert np.allclose(out[:, 1:], rgb / np.pi, rtol=1e-7)
+    rng = np.random.default_rng(5)
+    m = 200000
+    wo = rng.normal(size=(m, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    wo[:, 2] = np.abs(wo[:, 2])  # uniform on the upper hemisphere, pdf 1 / 2pi
+    q = np.zeros((m, 9), np.float32)
+    q[:, 5] = 1.0
+    q[:, 6:9] = wo
+    o = np.zeros((m, 4), np.float32)
+    orc.orc_lambert_eval(ptr(q), m, ptr(rgb), 0, ptr(o))
+    assert abs(o[:, 0].mean() * 2 * np.pi - 1.0) < 0.01
+
+
+def test_tent_filter(orc):
+    """The render loop's tent filter (demo_pt.cpp:689-697).  Its closed-form
+    points (-1 at 0, -0.5 at 1/8, 0 at 1/2, 0.5 at 7/8), the inverse-CDF
+    property F(tent(xi)) = xi of the triangular density 1 - |d| on
+    [-1, 1], and a chi-square test of the sub-pixel jitter the camera draws
+    per (pixel, sample) against that density, each axis and jointly
+    (independence)."""
+    xi = np.array([0.0, 0.125, 0.5, 0.875, 0.25, 0.75], np.float32)
+    out = np.zeros(xi.shape[0], np.float32)
+    orc.orc_tent(ptr(xi), xi.shape[0], ptr(out))
+    assert out.tolist() == [-1.0, -0.5, 0.0, 0.5, np.float32(np.sqrt(np.float32(0.5)) - 1), 1 - np.float32(np.sqrt(np.float32(0.5)))]
+    u = np.linspace(0, 1, 100001, dtype=np.float32)[:-1]
+    d = np.zeros(u.shape[0], np.float32)
+    orc.orc_tent(ptr(u), u.shape[0], ptr(d))
+    d64 = d.astype(np.float64)
+    F = np.where(d64 < 0, 0.5 * (1 + d64) ** 2, 1 - 0.5 * (1 - d64) ** 2)
+    assert np.max(np.abs(F - u)) < 3e-7
+    assert np.all(np.diff(d) >= 0) and d.min() >= -1 and d.max() < 1
+    n = 1 << 20
+    rng = np.random.default_rng(3)
+    pix = rng.integers(0, 1 << 20, n).astype(np.uint64)
+    smp = rng.integers(0, 64, n).astype(np.uint64)
+    j = np.zeros((n, 2), np.float32)
+    orc.orc_pixel_jitter(1, ptr(pix), ptr(smp), n, ptr(j))
+    edges = np.linspace(-1, 1, 41)
+    cdf = np.where(edges < 0, 0.5 * (1 + edges) ** 2, 1 - 0.5 * (1 - edges) ** 2)
+    p = np.diff(cdf)
+    from scipy.stats import chi2
+    for k in range(2):
+        c, _ = np.histogram(j[:, k], edges)
+        x2 = ((c - n * p) ** 2 / (n * p)).sum()
+        assert x2 < chi2.ppf(0.999, len(p) - 1), (k, x2)
+    e8 = np.linspace(-1, 1, 9)
+    c8 = np.where(e8 < 0, 0.5 * (1 + e8) ** 2, 1 - 0.5 * (1 - e8) ** 2)
+    p2 = np.outer(np.diff(c8), np.diff(c8)).ravel()
+    c2, _, _ = np.histogram2d(j[:, 0], j[:, 1], [e8, e8])
+    x2 = ((c2.ravel() - n * p2) ** 2 / (n * p2)).sum()
+    assert x2 < chi2.ppf(0.999, len(p2) - 1), x2
 
 
 def test_light_selection_vs_reference_fixture(orc):

@@ -50,8 +50,8 @@ def test_obj_roundtrip(tmp_path, orc):
     d = scenes.cornell_icosphere(1)
     path = os.path.join(tmp_path, "s.obj")
     scenes.write_obj(path, d.tris)
-    n = orc.orc_load_obj(path.encode(), None, None)
+    n = orc.orc_load_obj(path.encode(), None, None, None)
     t = np.zeros((n, 9), np.float32)
     vn = np.zeros((n, 9), np.float32)
-    orc.orc_load_obj(path.encode(), _libs.ptr(t), _libs.ptr(vn))
+    orc.orc_load_obj(path.encode(), None, _libs.ptr(t), _libs.ptr(vn))
     assert np.array_equal(t, d.tris)  # %.9g round-trips float32 exactly
