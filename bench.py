@@ -65,7 +65,7 @@ def cpu_baseline(scene_data, rays_np, sample):
     times = []
     for _ in range(3):
         t0 = time.time()
-        prim, _ = ob.intersect(sub, threads=cores)
+        prim, tuv = ob.intersect(sub, threads=cores)
         times.append(time.time() - t0)
     dt = sorted(times)[1]
     # the reference demo's own default: all cores but one (demo_pt.cpp:726)
@@ -77,7 +77,8 @@ def cpu_baseline(scene_data, rays_np, sample):
                       f"({tb:.1f} s, excluded) + recursive reference-order traversal, OpenMP over rays, "
                       "median of 3",
             "hit_fraction": round(float((prim >= 0).mean()), 4),
-            "reference_default_threads": {"cores": int(max(1, cores - 1)), "value": round(sample / dt1 / 1e6, 3)}}
+            "reference_default_threads": {"cores": int(max(1, cores - 1)), "value": round(sample / dt1 / 1e6, 3)}}, \
+        (prim, tuv)
 
 
 def timed_band(render, cores):
@@ -450,9 +451,18 @@ def main():
     }
     if not (args.no_cpu or args.profile):
         cb = cpu_baseline(data, rays_np, min(args.cpu_sample, n))
-        out["cpu_baseline"] = cb
         if cb:
+            cb, (oprim, otuv) = cb
+            # closest-hit parity of the timed launch's output with the oracle
+            # (the restated reference traversal) on the rays the CPU traced
+            m = oprim.shape[0]
+            gh = hits[:m].cpu().numpy()
+            mism = (gh[:, 0] != oprim) | np.any(gh[:, 1:].view(np.uint32) != otuv.view(np.uint32), axis=1)
+            out["hits_bit_exact"] = bool(not mism.any())
+            out["hits_compared"] = int(m)
+            out["hits_mismatched"] = int(mism.sum())
             out["gpu_over_cpu"] = round(mrays / cb["value"], 1)
+        out["cpu_baseline"] = cb
     if not args.no_render:
         del rays, hits
         torch.cuda.empty_cache()

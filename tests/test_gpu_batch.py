@@ -10,9 +10,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _scenes(ctx):
-    from mcrt_amd import scenes
-    specs = [  # (geodesic frequency, film w, h, render parameters)
+def _specs():
+    return [  # (geodesic frequency, film w, h, render parameters)
         (3, 40, 30, dict(spp=8, max_depth=6, rr_start=3, rr_prob=0.8, seed=1)),
         (9, 17, 23, dict(spp=3, max_depth=4, rr_start=1, rr_prob=0.7, seed=5)),
         (1, 64, 8, dict(spp=16, max_depth=8, rr_start=3, rr_prob=0.8, seed=1)),
@@ -20,11 +19,32 @@ def _scenes(ctx):
         (12, 24, 20, dict(spp=1, max_depth=6, rr_start=2, rr_prob=0.5, seed=2, sample_offset=7)),
         (4, 9, 50, dict(spp=11, max_depth=5, rr_start=3, rr_prob=0.9, seed=3)),
     ]
-    out = []
-    for k, (f, w, h, p) in enumerate(specs):
-        d = scenes.cornell_displaced(geodesic_freq=f, albedo_seed=40 + k, phong_frac=0.3, phong_exp=16.0)
-        out.append((d.to_scene(ctx, width=w, height=h), p))
-    return out
+
+
+def _data():
+    from mcrt_amd import scenes
+    return [(scenes.cornell_displaced(geodesic_freq=f, albedo_seed=40 + k, phong_frac=0.3, phong_exp=16.0), w, h, p)
+            for k, (f, w, h, p) in enumerate(_specs())]
+
+
+def _scenes(ctx):
+    return [(d.to_scene(ctx, width=w, height=h), p) for d, w, h, p in _data()]
+
+
+def test_batch_films_match_oracle(ctx):
+    """Each film of a batch (chunks cutting across scenes) equals the
+    oracle's fp32 integrator on that scene, bit for bit."""
+    import _libs
+    import mcrt_amd
+    data = _data()
+    sc = [d.to_scene(ctx, width=w, height=h) for d, w, h, _ in data]
+    sc[0].set_tuning(pt_chunk_paths=2500)
+    films = mcrt_amd.render_batch(sc, [p for *_, p in data])
+    for k, ((d, w, h, p), f) in enumerate(zip(data, films)):
+        o = _libs.OracleScene(d).render(w, h, threads=16, **p)
+        assert np.array_equal(o.view(np.uint32), f.view(np.uint32)), f"scene {k}"
+    for s in sc:
+        s.close()
 
 
 def test_batch_matches_individual_renders(ctx):
