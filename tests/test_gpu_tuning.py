@@ -79,6 +79,11 @@ def test_query_reorder_bit_exact(ctx, n):
         cam[:, 0:3] = rays[n // 2, 0:3]
         s.set_tuning(trace_reorder_rays=-1)
         want, want_cam = s.intersect(nan), s.intersect(cam)
+        # the NaN-origin ray (and every other) as the reference traversal gives it
+        onp, ontuv = _libs.OracleBVH(tris).intersect(nan)
+        wp, wtuv = mcrt_amd.split_hits(want)
+        assert np.array_equal(wp, onp) and np.array_equal(wtuv.view(np.uint32), ontuv.view(np.uint32))
+        assert wp[7] == -1
         s.set_tuning(trace_reorder_rays=1)
         assert np.array_equal(s.intersect(nan), want)
         assert np.array_equal(s.intersect(cam), want_cam)
