@@ -183,6 +183,21 @@ MCRT_API int mcrt_intersect(mcrt_scene* scene, const float* rays, size_t n, mcrt
                             int flags, void* stream);
 MCRT_API int mcrt_occluded(mcrt_scene* scene, const float* rays, size_t n, uint8_t* occluded,
                            int flags, void* stream);
+/* mcrt_intersect without allocating (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  rays and
+ * hits are device pointers (as with MCRT_DEVICE_PTRS) and the call is
+ * asynchronous on `stream`; results equal mcrt_intersect's bit for bit.
+ * The launch's work counters and traversal stack spill area live in a
+ * 64 MiB array of device memory that each GPU the library runs on holds
+ * from mcrt_create until the process ends; the ray reorder of large
+ * queries is not used.  A scene whose deepest traversal stack needs more
+ * spill per wavefront than the array holds is rejected (MCRT_ERR_INVALID_ARG);
+ * deep stacks otherwise run on fewer resident waves.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (several streams or host
+ * threads) race and must be serialised by the caller. */
+MCRT_API int mcrt_intersect_noalloc(mcrt_scene* scene, const float* rays, size_t n, mcrt_hit* hits,
+                                    void* stream);
 /* Instrumented query (no reference counterpart; measurement only): the
  * same traversal as mcrt_intersect (any_hit = 0) or mcrt_occluded
  * (any_hit = 1), writing no results, accumulating into counters[4] the

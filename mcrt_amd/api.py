@@ -89,6 +89,7 @@ def lib():
         "mcrt_scene_num_triangles": (I, [P, ctypes.POINTER(Z)]),
         "mcrt_intersect": (I, [P, P, Z, P, I, P]),
         "mcrt_occluded": (I, [P, P, Z, P, I, P]),
+        "mcrt_intersect_noalloc": (I, [P, P, Z, P, P]),
         "mcrt_query_counters": (I, [P, P, Z, I, P, I, P]),
         "mcrt_render": (I, [P, ctypes.POINTER(RenderParams), P, I, P]),
         "mcrt_render_adjoint": (I, [P, ctypes.POINTER(RenderParams), P, P, P, P, I, P]),

@@ -115,6 +115,13 @@ MCRT_API int mcrt_create(int device, mcrt_ctx** out)
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
     c->name = prop.gcnArchName;
+    // the allocation-free query's scratch array on this device (the device
+    // is current); a failure here only disables that entry point
+    if (scratch_address(device, &c->scratch) != MCRT_OK)
+    {
+        c->scratch = nullptr;
+        c->scratch_err = mcrt_last_error();
+    }
     *out = c;
     return MCRT_OK;
 }
@@ -334,6 +341,43 @@ MCRT_API int mcrt_intersect(mcrt_scene* s, const float* rays, size_t n, mcrt_hit
     if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_intersect: NULL scene");
     if (n && (!rays || !hits)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_intersect: NULL buffer");
     return run_query(s, rays, n, hits, nullptr, TRACE_CLOSEST, flags, (hipStream_t)stream, nullptr);
+}
+
+MCRT_API int mcrt_intersect_noalloc(mcrt_scene* s, const float* rays, size_t n, mcrt_hit* hits, void* stream)
+{
+    if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_intersect_noalloc: NULL scene");
+    if (n && (!rays || !hits)) return fail(MCRT_ERR_INVALID_ARG, "mcrt_intersect_noalloc: NULL buffer");
+    if (!s->finalized) return fail(MCRT_ERR_STATE, "mcrt_intersect_noalloc: scene not finalized");
+    if (!s->ctx->scratch)
+        return fail(MCRT_ERR_DEVICE, "mcrt_intersect_noalloc: no scratch memory: %s", s->ctx->scratch_err.c_str());
+    if (n == 0) return MCRT_OK;
+    HIPCHK(hipSetDevice(s->ctx->device));
+    // the scene's launch geometry, with the work counters and the stack spill
+    // area in the scratch array and no ray reorder (results do not depend on
+    // the order; the reorder's buffers would be allocations)
+    DevScene view = s->dev;
+    char* base = static_cast<char*>(s->ctx->scratch);
+    view.wq = reinterpret_cast<uint32_t*>(base);
+    view.order = nullptr;
+    view.cell_work = nullptr;
+    view.reorder_rays = 0;
+    view.spill = nullptr;
+    if (view.spill_depth > 0)
+    {
+        // fewer resident lanes when the spill area would not fit: waves claim
+        // rays from the work counters, so any grid covers every ray
+        const size_t per_lane = (size_t)view.spill_depth * sizeof(uint64_t);
+        const int64_t lanes = (int64_t)((kScratchBytes - kScratchWqBytes) / per_lane) / 64 * 64;
+        if (lanes < 64)
+            return fail(MCRT_ERR_INVALID_ARG,
+                        "mcrt_intersect_noalloc: the scene's traversal stack needs %zu bytes of spill per "
+                        "wavefront, more than the %zu bytes of scratch memory",
+                        per_lane * 64, kScratchBytes - kScratchWqBytes);
+        if (lanes < view.spill_lanes) view.spill_lanes = lanes;
+        view.spill = reinterpret_cast<uint64_t*>(base + kScratchWqBytes);
+    }
+    HIPCHK(launch_trace(view, rays, (int64_t)n, hits, nullptr, TRACE_CLOSEST, nullptr, (hipStream_t)stream));
+    return MCRT_OK;
 }
 
 MCRT_API int mcrt_occluded(mcrt_scene* s, const float* rays, size_t n, uint8_t* occluded, int flags, void* stream)

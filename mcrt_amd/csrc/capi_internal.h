@@ -99,6 +99,10 @@ struct mcrt_ctx
     int device = 0;
     int num_cus = 0;
     std::string name;
+    // device address of the allocation-free query's scratch array on this
+    // context's device (nullptr: unavailable, scratch_err says why)
+    void* scratch = nullptr;
+    std::string scratch_err;
     // per-stream spill stacks and path-tracer workspaces, shared by the
     // scenes of the context (work on one stream runs in order)
     std::mutex res_mu;

@@ -115,6 +115,19 @@ enum TraceMode
 hipError_t launch_trace(const DevScene& s, const float* rays, int64_t n, void* hits, uint8_t* occ,
                         int mode, uint64_t* counters, hipStream_t stream);
 
+// Device memory of the allocation-free closest-hit query
+// (mcrt_intersect_noalloc): a file-scope array in traverse.hip, one copy per
+// GPU, holding the launch's work counters (the first kScratchWqBytes) and
+// then its traversal stack spill area.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr size_t kScratchWqBytes = 512;
+static_assert(kWqWords * sizeof(uint32_t) <= kScratchWqBytes, "work counters fit their scratch slot");
+
+// The device address of that array on `device` (which must be current),
+// looked up once per device; 0 = MCRT_OK, else a status with the message
+// set.
+int scratch_address(int device, void** out);
+
 // LDS stack depths the traversal kernels are instantiated for.
 inline bool valid_lds_depth(int d) { return d == 8 || d == 12 || d == 16 || d == 24; }
 }  // namespace mcrt

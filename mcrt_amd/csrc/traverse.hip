@@ -1,10 +1,19 @@
 // Ray-query launches (mcrt_intersect / mcrt_occluded / mcrt_query_counters).
+#include <map>
+#include <mutex>
+
+#include "capi_internal.h"
 #include "trace_kernel.h"
 
 namespace mcrt
 {
 namespace
 {
+// The allocation-free query's device memory (kScratchBytes, device.h).  In
+// host code `scratch` is only a stand-in for the symbol: its device address
+// comes from hipGetSymbolAddress (scratch_address), per GPU.
+static __device__ __align__(256) uint64_t scratch[kScratchBytes / sizeof(uint64_t)];
+
 __global__ void k_wq_reset(uint32_t* wq)
 {
     if (threadIdx.x < kSlices) wq[threadIdx.x * kSliceStride] = 0;
@@ -180,6 +189,29 @@ hipError_t launch(const DevScene& s, const float* rays, int64_t n, void* hits, u
     return hipGetLastError();
 }
 }  // namespace
+
+int scratch_address(int device, void** out)
+{
+    static std::mutex mu;
+    static std::map<int, void*> addr;  // device ordinal -> address of `scratch` there
+    std::lock_guard<std::mutex> g(mu);
+    const auto it = addr.find(device);
+    if (it != addr.end())
+    {
+        *out = it->second;
+        return MCRT_OK;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+    {
+        (void)hipGetLastError();
+        return fail(MCRT_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    }
+    addr[device] = p;
+    *out = p;
+    return MCRT_OK;
+}
 
 hipError_t launch_trace(const DevScene& s, const float* rays, int64_t n, void* hits, uint8_t* occ, int mode,
                         uint64_t* counters, hipStream_t stream)
