@@ -35,17 +35,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic():
-    """HBM bytes per traversal launch measured by rocprofv3 (FETCH_SIZE and
-    WRITE_SIZE scaled by a copy-kernel calibration) for this workload."""
+def profiled_figures():
+    """Figures of the headline kernel measured by rocprofv3 for this
+    workload, from the committed profile: HBM bytes per launch (FETCH_SIZE
+    and WRITE_SIZE scaled by a copy-kernel calibration) and its vector
+    issue (`key: value` lines)."""
+    out = {}
     try:
         with open(PROFILE) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return int(line.split(":")[1])
+                for key, conv in (("traffic_bytes_per_launch", int), ("valu_issue_frac", float),
+                                  ("valu_insts_per_ray", float)):
+                    if line.startswith(key + ":"):
+                        out[key] = conv(line.split(":")[1])
     except OSError:
         pass
-    return None
+    return out
 
 
 def cpu_baseline(scene_data, rays_np, sample):
@@ -287,8 +292,17 @@ def c3_leg(ctx, steps):
             ms = e0.elapsed_time(e1) / steps
             key = f"{kind}_{'any_hit' if any_hit else 'closest'}"
             frac = float(occ.float().mean()) if any_hit else float((hits[:, 0] >= 0).float().mean())
+            # algorithmic bytes of this leg from its own visit counts (the
+            # instrumented traversal, same tree, same order, same early exit)
+            cnt = s.query_counters(rays, any_hit=any_hit)
+            per = {k: cnt[k] / cnt["rays"] for k in ("nodes", "refs", "tris")}
+            bpr = RAY_B + (1 if any_hit else HIT_B) + NODE_B * per["nodes"] + REF_B * per["refs"] + TRI_B * per["tris"]
+            ach = bpr * n / (ms * 1e-3) / 1e9
             out[key] = {"mrays_per_s": round(n / ms / 1e3, 1), "ms": round(ms, 3),
-                        ("occluded_fraction" if any_hit else "hit_fraction"): round(frac, 4)}
+                        ("occluded_fraction" if any_hit else "hit_fraction"): round(frac, 4),
+                        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_ray": round(bpr, 1),
+                                     "visits_per_ray": {k: round(v, 3) for k, v in per.items()}}}
             del rays
     s.close()
     return out
@@ -298,8 +312,9 @@ def c4_leg(ctx, steps):
     """Batched uneven rendering: 64 Cornell + displaced-sphere scenes
     (1k..500k triangles, film side 128..512, 8..64 spp, depth 6, seed 21),
     rendered plainly (one mcrt_render per scene, in order) and batched (one
-    hipGraph replay of the whole batch); Msamples/s of each and whether the
-    64 films of the two forms are bit-identical."""
+    hipGraph replay of the whole batch, its wavefront settings chosen first
+    by mcrt_batch_autotune); Msamples/s of each and whether the 64 films of
+    the two forms are bit-identical."""
     import torch
     import mcrt_amd
     from mcrt_amd import scenes
@@ -333,6 +348,9 @@ def c4_leg(ctx, steps):
     for f in films:
         f.fill_(-1.0)
     b = mcrt_amd.Batch(sc, params, films, graph=True)
+    t0 = time.time()
+    tuned, tuned_ms = b.autotune()  # wavefront settings of the batch, before its graph is captured
+    t_tune = time.time() - t0
     t_batch = timed(b.render)
     same = all(bool(torch.equal(a.view(torch.int32), r.view(torch.int32))) for a, r in zip(films, ref))
     info = b.info()
@@ -343,8 +361,11 @@ def c4_leg(ctx, steps):
             "plain_ms": round(t_plain, 2), "plain_msamples_per_s": round(nsamp / t_plain / 1e3, 2),
             "batched_ms": round(t_batch, 2), "batched_msamples_per_s": round(nsamp / t_batch / 1e3, 2),
             "batched_over_plain": round(t_plain / t_batch, 3), "films_bit_identical": same,
+            "autotune": {"seconds": round(t_tune, 1), "best_ms": round(tuned_ms, 2),
+                         "tuning": {k: tuned[k] for k in ("pt_trace_waves_per_cu", "pt_trace_lds_depth",
+                                                          "pt_trace_refill_lanes", "pt_chunk_paths")}},
             "config": {"triangles": int(sum(d.ntri for d, _, _ in batch)), "depth": scenes.C4_RENDER["max_depth"],
-                       "seed": 21, "graph": True}}
+                       "seed": 21, "graph": True, "autotuned": True}}
 
 
 def main():
@@ -360,6 +381,9 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the 64-scene batched rendering leg")
     ap.add_argument("--no-env", action="store_true", help="skip the environment-lighting render leg")
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
+    ap.add_argument("--legs", default=None,
+                    help="comma-separated subset of main,render,c3,c4,env,adjoint (default: all; with --profile "
+                         "main,render,adjoint)")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (counter calibration)")
@@ -384,14 +408,52 @@ def main():
     import mcrt_amd
     from mcrt_amd import scenes
 
-    t0 = time.time()
-    data = scenes.cornell_displaced(8)
-    log(f"scene: {data.ntri} triangles generated in {time.time() - t0:.1f} s")
+    if args.legs is None:
+        legs = {"main", "render", "adjoint"} if args.profile else {"main", "render", "c3", "c4", "env", "adjoint"}
+    else:
+        legs = set(args.legs.split(","))
+    for flag, leg in (("no_render", "render"), ("no_adjoint", "adjoint"), ("no_c3", "c3"), ("no_c4", "c4"),
+                      ("no_env", "env")):
+        if getattr(args, flag):
+            legs.discard(leg)
     ctx = mcrt_amd.Context(0)
-    scene = data.to_scene(ctx)
-    st = scene.stats
-    log(f"SBVH: {st['build_seconds']:.2f} s build, {st['refs']} refs, {st['wide_nodes']} wide nodes")
+    out = {}
+    data = scene = None
+    if legs & {"main", "render"}:
+        t0 = time.time()
+        data = scenes.cornell_displaced(8)
+        log(f"scene: {data.ntri} triangles generated in {time.time() - t0:.1f} s")
+        scene = data.to_scene(ctx)
+        st = scene.stats
+        log(f"SBVH: {st['build_seconds']:.2f} s build, {st['refs']} refs, {st['wide_nodes']} wide nodes")
+    if "main" in legs:
+        out = main_leg(args, scene, data)
+    if "render" in legs:
+        torch.cuda.empty_cache()
+        r = render_leg(scene, data, 1 if args.profile else 2, 1, cpu=not (args.no_cpu or args.profile))
+        out["render"] = r
+        out["msamples_per_s"] = r["msamples_per_s"]
+    if scene is not None:
+        scene.close()
+    if "c3" in legs:
+        out["c3_soup"] = c3_leg(ctx, 1 if args.profile else 5)
+    if "c4" in legs:
+        out["c4_batch"] = c4_leg(ctx, 1 if args.profile else 2)
+    if "env" in legs:
+        out["env_render"] = env_leg(ctx, 1 if args.profile else 2, cpu=not (args.no_cpu or args.profile))
+    if "adjoint" in legs:
+        a = adjoint_leg(ctx, 1 if args.profile else 3)
+        out["adjoint"] = a
+        out["adjoint_forward_ratio"] = a["adjoint_forward_ratio"]
+    print(json.dumps(out), flush=True)
 
+
+def main_leg(args, scene, data):
+    """The headline: closest hit of the 16M incoherent rays, resident in
+    HBM, timed with HIP events on the launch stream."""
+    import torch
+    from mcrt_amd import scenes
+    st = scene.stats
     n = args.rays
     rays_np = scenes.incoherent_rays(n, lo=BOX_LO, hi=BOX_HI, seed=3)
     rays = torch.from_numpy(rays_np).cuda()
@@ -422,6 +484,7 @@ def main():
 
     achieved = bytes_per_ray * n / (ms_step * 1e-3) / 1e9
     tune = scene.tuning()
+    prof = profiled_figures() if n == 1 << 24 else {}
     out = {
         "metric": "Mrays/s closest-hit (incoherent, 1.3M-tri displaced icosphere in Cornell box)",
         "value": round(mrays, 1),
@@ -441,7 +504,7 @@ def main():
                    "sbvh_build_s": round(st["build_seconds"], 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic() if n == 1 << 24 else None,
+                     "traffic": prof.get("traffic_bytes_per_launch"),
                      "traffic_source": os.path.relpath(PROFILE, ROOT),
                      "bytes_per_ray": round(bytes_per_ray, 1),
                      "visits_per_ray": {k: round(v, 3) for k, v in per_ray.items()},
@@ -449,6 +512,13 @@ def main():
                                                                           tune["trace_waves_per_cu"])},
         "cpu_baseline": None,
     }
+    if "valu_issue_frac" in prof:
+        # what limits the kernel, measured: the fraction of the CUs' vector
+        # issue slots it uses (the algorithmic-byte fraction above is the
+        # contract's roofline; the traffic beyond L2 is well below HBM peak)
+        out["roofline"]["measured_limit"] = {"kind": "vector issue", "valu_issue_frac": prof["valu_issue_frac"],
+                                             "valu_insts_per_ray": prof.get("valu_insts_per_ray"),
+                                             "source": os.path.relpath(PROFILE, ROOT)}
     if not (args.no_cpu or args.profile):
         cb = cpu_baseline(data, rays_np, min(args.cpu_sample, n))
         if cb:
@@ -463,23 +533,8 @@ def main():
             out["hits_mismatched"] = int(mism.sum())
             out["gpu_over_cpu"] = round(mrays / cb["value"], 1)
         out["cpu_baseline"] = cb
-    if not args.no_render:
-        del rays, hits
-        torch.cuda.empty_cache()
-        r = render_leg(scene, data, 1 if args.profile else 2, 1, cpu=not (args.no_cpu or args.profile))
-        out["render"] = r
-        out["msamples_per_s"] = r["msamples_per_s"]
-    if not (args.no_c3 or args.profile):
-        out["c3_soup"] = c3_leg(ctx, 5)
-    if not (args.no_c4 or args.profile):
-        out["c4_batch"] = c4_leg(ctx, 2)
-    if not (args.no_env or args.profile):
-        out["env_render"] = env_leg(ctx, 2, cpu=not args.no_cpu)
-    if not args.no_adjoint:
-        a = adjoint_leg(ctx, 1 if args.profile else 3)
-        out["adjoint"] = a
-        out["adjoint_forward_ratio"] = a["adjoint_forward_ratio"]
-    print(json.dumps(out), flush=True)
+    del rays, hits
+    return out
 
 
 if __name__ == "__main__":

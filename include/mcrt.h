@@ -289,23 +289,42 @@ typedef struct mcrt_tuning {
                                       -1: never.  Unset, finalize picks 2^20 when
                                       the BVH exceeds 192 MB (larger than the GPU's
                                       last-level cache), else -1                 */
+    int32_t trace_refill_lanes;    /* compaction threshold of the queries' traversal,
+                                      1..64 (16): a wave hands new rays to its idle
+                                      lanes once at least this many of its 64 lanes
+                                      are idle, so refills come in fuller rounds    */
+    int32_t pt_trace_refill_lanes; /* the same for the traversal launches of renders (16) */
 } mcrt_tuning;
 MCRT_API int mcrt_scene_set_tuning(mcrt_scene* scene, const mcrt_tuning* tuning);
 MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* scene, mcrt_tuning* tuning);
 
 /* Autotuner (no reference counterpart): times closest-hit over the n rays
  * for trace_waves_per_cu in {8,16,24,32,64,128} x trace_lds_depth in
- * {8,12,16} and keeps the fastest, then trace_reorder_rays -1 against
- * min(n, 2^20) (the caller's ray order against origin-cell order) with the
- * chosen pair, and keeps the faster; with `render` != NULL also times one
- * render per pt_trace_waves_per_cu in {16,32,64} x pt_trace_lds_depth in
- * {8,16}, then per pt_chunk_paths in 2^22..2^26, keeping the fastest.  The
- * chosen tuning is applied to the scene and returned in `best`, the best
- * closest-hit rate in `best_mrays_per_s` (either may be NULL).  Rays follow
- * `flags` like the queries; synchronous. */
+ * {8,12,16} and keeps the fastest, then trace_refill_lanes in {1,8,16,32}
+ * with the chosen pair, then trace_reorder_rays -1 against min(n, 2^20)
+ * (the caller's ray order against origin-cell order), keeping the fastest
+ * each time; with `render` != NULL also times one render per
+ * pt_trace_waves_per_cu in {16,32,64} x pt_trace_lds_depth in {8,16}, then
+ * per pt_trace_refill_lanes in {1,8,16,32}, then per pt_chunk_paths in
+ * 2^22..2^26, keeping the fastest.  The chosen tuning is applied to the
+ * scene and returned in `best`, the best closest-hit rate in
+ * `best_mrays_per_s` (either may be NULL); a failure restores the scene's
+ * tuning.  Rays follow `flags` like the queries; synchronous. */
 MCRT_API int mcrt_autotune(mcrt_scene* scene, const float* rays, size_t n,
                            const mcrt_render_params* render, int flags, void* stream,
                            mcrt_tuning* best, double* best_mrays_per_s);
+/* Autotuner of a batch (no reference counterpart): the render half of
+ * mcrt_autotune timed on the whole batch -- pt_trace_waves_per_cu in
+ * {16,32,64} x pt_trace_lds_depth in {8,16}, then pt_trace_refill_lanes in
+ * {1,8,16,32}, then pt_chunk_paths in 2^22..2^26, one batch render per
+ * setting into the batch's films, keeping the fastest each time.  The
+ * chosen tuning is applied to scenes[0] (whose tuning the batch uses) and
+ * the batch re-initialised with it; a captured graph is dropped and
+ * captured again by the next mcrt_batch_render.  Returns the tuning in
+ * `best` and the fastest render's milliseconds in `best_ms` (either may be
+ * NULL).  Waits for the device first; synchronous.  A failure restores
+ * scenes[0]'s tuning. */
+MCRT_API int mcrt_batch_autotune(mcrt_batch* batch, void* stream, mcrt_tuning* best, double* best_ms);
 
 #ifdef __cplusplus
 }

@@ -50,7 +50,8 @@ class RenderParams(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("trace_waves_per_cu", ctypes.c_int32), ("trace_lds_depth", ctypes.c_int32),
                 ("pt_chunk_paths", ctypes.c_int64), ("pt_trace_waves_per_cu", ctypes.c_int32),
-                ("pt_trace_lds_depth", ctypes.c_int32), ("trace_reorder_rays", ctypes.c_int64)]
+                ("pt_trace_lds_depth", ctypes.c_int32), ("trace_reorder_rays", ctypes.c_int64),
+                ("trace_refill_lanes", ctypes.c_int32), ("pt_trace_refill_lanes", ctypes.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -98,6 +99,7 @@ def lib():
         "mcrt_film_tonemap_gamma": (I, [P, Z, ctypes.c_float]),
         "mcrt_batch_create": (I, [Z, P, ctypes.POINTER(RenderParams), P, I, ctypes.POINTER(P)]),
         "mcrt_batch_render": (I, [P, P]),
+        "mcrt_batch_autotune": (I, [P, P, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_double)]),
         "mcrt_batch_info": (I, [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
         "mcrt_batch_destroy": (I, [P]),
         "mcrt_render_batch": (I, [Z, P, ctypes.POINTER(RenderParams), P, I, P]),
@@ -323,11 +325,11 @@ class Scene:
         return dict(zip(("nodes", "refs", "tris", "rays"), (int(x) for x in c)))
 
     def set_tuning(self, trace_waves_per_cu=0, trace_lds_depth=0, pt_chunk_paths=0, pt_trace_waves_per_cu=0,
-                   pt_trace_lds_depth=0, trace_reorder_rays=0):
+                   pt_trace_lds_depth=0, trace_reorder_rays=0, trace_refill_lanes=0, pt_trace_refill_lanes=0):
         """Launch tuning (0 keeps a field; trace_reorder_rays=-1: never
         reorder queries); results never depend on it."""
         t = Tuning(trace_waves_per_cu, trace_lds_depth, pt_chunk_paths, pt_trace_waves_per_cu, pt_trace_lds_depth,
-                   trace_reorder_rays)
+                   trace_reorder_rays, trace_refill_lanes, pt_trace_refill_lanes)
         _check(lib().mcrt_scene_set_tuning(self.h, ctypes.byref(t)))
 
     def tuning(self):
@@ -399,6 +401,16 @@ class Batch:
         paths, chunks = ctypes.c_int64(), ctypes.c_int64()
         _check(lib().mcrt_batch_info(self.h, ctypes.byref(paths), ctypes.byref(chunks)))
         return {"paths": paths.value, "chunks": chunks.value}
+
+    def autotune(self):
+        """Sweeps the render tuning on this batch (renders it several times
+        into its films) and applies the fastest to its first scene; a
+        captured graph is captured again by the next render.  Returns
+        (tuning dict, fastest render in ms)."""
+        t = Tuning()
+        ms = ctypes.c_double()
+        _check(lib().mcrt_batch_autotune(self.h, _stream_of(True), ctypes.byref(t), ctypes.byref(ms)))
+        return t.as_dict(), ms.value
 
     def close(self):
         if self.h:

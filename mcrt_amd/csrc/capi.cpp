@@ -492,6 +492,10 @@ static int check_tuning(const mcrt_tuning* t)
         return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: pt_trace_lds_depth must be 8, 12, 16 or 24 (0 = keep)");
     if (t->trace_reorder_rays < -1)
         return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_reorder_rays is -1 (never) or a ray count (0 = keep)");
+    if (t->trace_refill_lanes < 0 || t->trace_refill_lanes > 64)
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_refill_lanes in [1, 64] (0 = keep)");
+    if (t->pt_trace_refill_lanes < 0 || t->pt_trace_refill_lanes > 64)
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: pt_trace_refill_lanes in [1, 64] (0 = keep)");
     return MCRT_OK;
 }
 
@@ -506,6 +510,8 @@ MCRT_API int mcrt_scene_set_tuning(mcrt_scene* s, const mcrt_tuning* t)
     if (t->pt_trace_waves_per_cu) s->pt_trace_waves_per_cu = t->pt_trace_waves_per_cu;
     if (t->pt_trace_lds_depth) s->pt_trace_lds_depth = t->pt_trace_lds_depth;
     if (t->trace_reorder_rays) s->trace_reorder_rays = t->trace_reorder_rays;
+    if (t->trace_refill_lanes) s->trace_refill_lanes = t->trace_refill_lanes;
+    if (t->pt_trace_refill_lanes) s->pt_trace_refill_lanes = t->pt_trace_refill_lanes;
     if (s->finalized) apply_trace_tuning(s);
     return MCRT_OK;
 }
@@ -519,6 +525,8 @@ MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* s, mcrt_tuning* t)
     t->pt_trace_waves_per_cu = s->pt_trace_waves_per_cu;
     t->pt_trace_lds_depth = s->pt_trace_lds_depth;
     t->trace_reorder_rays = s->trace_reorder_rays;
+    t->trace_refill_lanes = s->trace_refill_lanes;
+    t->pt_trace_refill_lanes = s->pt_trace_refill_lanes;
     return MCRT_OK;
 }
 
@@ -542,6 +550,8 @@ MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcr
         s->pt_trace_waves_per_cu = before.pt_trace_waves_per_cu;
         s->pt_trace_lds_depth = before.pt_trace_lds_depth;
         s->trace_reorder_rays = before.trace_reorder_rays;
+        s->trace_refill_lanes = before.trace_refill_lanes;
+        s->pt_trace_refill_lanes = before.pt_trace_refill_lanes;
         if (s->finalized) apply_trace_tuning(s);
         fail(rc, "%s", err.c_str());
     }
@@ -610,6 +620,32 @@ static int autotune(mcrt_scene* s, const float* rays, size_t n, const mcrt_rende
             }
         s->trace_waves_per_cu = bw;
         s->trace_lds_depth = bd;
+        // then the compaction threshold with the chosen pair
+        static const int kRefill[] = {1, 8, 16, 32};
+        int br = s->trace_refill_lanes;
+        double refill_rate = 0.0;
+        for (int k : kRefill)
+        {
+            s->trace_refill_lanes = k;
+            apply_trace_tuning(s);
+            DevScene view;
+            if ((rc = launch_view(s, res, view, (int64_t)n))) return rc;
+            HIPCHK(launch_trace(view, r, (int64_t)n, d_hits.p, nullptr, TRACE_CLOSEST, nullptr, st));
+            HIPCHK(hipEventRecord(e0, st));
+            for (int j = 0; j < 3; j++)
+                HIPCHK(launch_trace(view, r, (int64_t)n, d_hits.p, nullptr, TRACE_CLOSEST, nullptr, st));
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            const double rate = 3.0 * (double)n / (ms * 1e-3) / 1e6;
+            if (rate > refill_rate)
+            {
+                refill_rate = rate;
+                br = k;
+            }
+        }
+        s->trace_refill_lanes = br;
         // then the ray order: the caller's, or by origin cell from min(n, 2^20) rays
         const int64_t order[2] = {-1, (int64_t)n < (int64_t(1) << 20) ? (int64_t)n : int64_t(1) << 20};
         double order_rate[2] = {0.0, 0.0};
@@ -670,6 +706,21 @@ static int autotune(mcrt_scene* s, const float* rays, size_t n, const mcrt_rende
             }
         s->pt_trace_waves_per_cu = pw;
         s->pt_trace_lds_depth = pd;
+        static const int kPtRefill[] = {1, 8, 16, 32};
+        int pr = s->pt_trace_refill_lanes;
+        best_ms = 0.0f;
+        for (int k : kPtRefill)
+        {
+            s->pt_trace_refill_lanes = k;
+            float ms = 0.0f;
+            if ((rc = time_render(ms))) return rc;
+            if (best_ms == 0.0f || ms < best_ms)
+            {
+                best_ms = ms;
+                pr = k;
+            }
+        }
+        s->pt_trace_refill_lanes = pr;
         int64_t bc = s->pt_chunk_paths;
         best_ms = 0.0f;
         for (int lg = 22; lg <= 26; lg++)
@@ -762,6 +813,117 @@ MCRT_API int mcrt_batch_info(const mcrt_batch* b, int64_t* total_paths, int64_t*
     if (!b) return fail(MCRT_ERR_INVALID_ARG, "mcrt_batch_info: NULL batch");
     if (total_paths) *total_paths = b->total_paths;
     if (chunks) *chunks = (int64_t)b->chunks.size();
+    return MCRT_OK;
+}
+
+// The render half of mcrt_autotune on a batch: each setting applied to the
+// batch's first scene (whose tuning a batch uses), the batch re-initialised
+// (chunks, workspace) and rendered once, timed with events on `st`.
+static int batch_autotune(mcrt_batch* b, hipStream_t st, double* best_ms_out)
+{
+    mcrt_scene* s0 = b->scenes[0];
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    struct Ev
+    {
+        hipEvent_t a, b;
+        ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } ev{e0, e1};
+    int rc;
+    auto time_batch = [&](float& ms) -> int {
+        int e = batch_init(b);
+        if (e) return e;
+        HIPCHK(hipEventRecord(e0, st));
+        if ((e = batch_enqueue(b, st))) return e;
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        return MCRT_OK;
+    };
+    float ms = 0.0f, best_ms = 0.0f;
+    if ((rc = time_batch(ms))) return rc;  // warm-up: code objects, workspace
+    static const int kPtWaves[] = {16, 32, 64};
+    static const int kPtDepth[] = {8, 16};
+    int pw = s0->pt_trace_waves_per_cu, pd = s0->pt_trace_lds_depth;
+    for (int w : kPtWaves)
+        for (int d : kPtDepth)
+        {
+            s0->pt_trace_waves_per_cu = w;
+            s0->pt_trace_lds_depth = d;
+            if ((rc = time_batch(ms))) return rc;
+            if (best_ms == 0.0f || ms < best_ms)
+            {
+                best_ms = ms;
+                pw = w;
+                pd = d;
+            }
+        }
+    s0->pt_trace_waves_per_cu = pw;
+    s0->pt_trace_lds_depth = pd;
+    static const int kPtRefill[] = {1, 8, 16, 32};
+    int pr = s0->pt_trace_refill_lanes;
+    best_ms = 0.0f;
+    for (int k : kPtRefill)
+    {
+        s0->pt_trace_refill_lanes = k;
+        if ((rc = time_batch(ms))) return rc;
+        if (best_ms == 0.0f || ms < best_ms)
+        {
+            best_ms = ms;
+            pr = k;
+        }
+    }
+    s0->pt_trace_refill_lanes = pr;
+    int64_t bc = s0->pt_chunk_paths;
+    best_ms = 0.0f;
+    for (int lg = 22; lg <= 26; lg++)
+    {
+        const int64_t c = int64_t(1) << lg;
+        if (lg > 22 && (c >> 1) >= b->total_paths) break;  // larger chunks change nothing
+        s0->pt_chunk_paths = c;
+        if ((rc = time_batch(ms))) return rc;
+        if (best_ms == 0.0f || ms < best_ms)
+        {
+            best_ms = ms;
+            bc = c;
+        }
+    }
+    s0->pt_chunk_paths = bc;
+    if ((rc = batch_init(b))) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    if (best_ms_out) *best_ms_out = best_ms;
+    return MCRT_OK;
+}
+
+MCRT_API int mcrt_batch_autotune(mcrt_batch* b, void* stream, mcrt_tuning* best, double* best_ms)
+{
+    if (!b) return fail(MCRT_ERR_INVALID_ARG, "mcrt_batch_autotune: NULL batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> g(b->res.mu);
+    // a captured graph holds the old settings: drop it (after any replay in
+    // flight), the next mcrt_batch_render captures the tuned sequence
+    HIPCHK(hipDeviceSynchronize());
+    if (b->exec) HIPCHK(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+    if (b->graph) HIPCHK(hipGraphDestroy(b->graph));
+    b->graph = nullptr;
+    mcrt_scene* s0 = b->scenes[0];
+    mcrt_tuning before;
+    mcrt_scene_get_tuning(s0, &before);
+    const int rc = batch_autotune(b, st, best_ms);
+    if (rc)
+    {
+        const std::string err = mcrt_last_error();
+        s0->pt_trace_waves_per_cu = before.pt_trace_waves_per_cu;
+        s0->pt_trace_lds_depth = before.pt_trace_lds_depth;
+        s0->pt_trace_refill_lanes = before.pt_trace_refill_lanes;
+        s0->pt_chunk_paths = before.pt_chunk_paths;
+        (void)batch_init(b);
+        return fail(rc, "%s", err.c_str());
+    }
+    if (best) mcrt_scene_get_tuning(s0, best);
     return MCRT_OK;
 }
 

@@ -43,7 +43,7 @@ struct DevScene
     int32_t stack_bound = 0;
     const int32_t* hot_tri = nullptr;  // triangle of each gradient hot slot
     int32_t nhot = 0;
-    int32_t pad0 = 0;
+    int32_t refill_lanes = 1;          // a traversal wave claims new rays once this many of its lanes are idle
     int64_t ntri = 0;
     uint64_t* spill = nullptr;         // traversal stack overflow storage
     int64_t spill_lanes = 0;           // lanes the spill buffer covers = resident lanes of a launch

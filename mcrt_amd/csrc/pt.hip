@@ -1064,6 +1064,7 @@ int batch_init(mcrt_batch* b)
     Dt.refs = nullptr;
     Dt.stack_bound = max_sb;
     trace_geometry(Dt, b->ctx->num_cus, s0->pt_trace_waves_per_cu, s0->pt_trace_lds_depth);
+    Dt.refill_lanes = s0->pt_trace_refill_lanes;
     if (Dt.spill_depth > 0)
     {
         if ((rc = b->res.spill.ensure((size_t)Dt.spill_lanes * Dt.spill_depth * sizeof(uint64_t)))) return rc;

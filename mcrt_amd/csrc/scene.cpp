@@ -615,6 +615,7 @@ void apply_trace_tuning(mcrt_scene* s)
 {
     trace_geometry(s->dev, s->ctx->num_cus, s->trace_waves_per_cu, s->trace_lds_depth);
     s->dev.reorder_rays = s->trace_reorder_rays > 0 ? s->trace_reorder_rays : 0;
+    s->dev.refill_lanes = s->trace_refill_lanes;
 }
 
 StreamRes* stream_res(mcrt_scene* s, hipStream_t stream)
@@ -630,6 +631,7 @@ int render_view(mcrt_scene* s, StreamRes* r, DevScene& view)
     int rc = launch_view(s, r, view);
     if (rc) return rc;
     trace_geometry(view, s->ctx->num_cus, s->pt_trace_waves_per_cu, s->pt_trace_lds_depth);
+    view.refill_lanes = s->pt_trace_refill_lanes;
     if (view.spill_depth > 0)
     {
         if ((rc = r->spill.ensure((size_t)view.spill_lanes * view.spill_depth * sizeof(uint64_t)))) return rc;

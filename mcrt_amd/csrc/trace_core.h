@@ -158,6 +158,8 @@ struct TraceStack
         node[k * stride + lane] = n;
         lo[k * stride + lane] = l;
     }
+    // The same without the entry distance (any-hit never re-tests it).
+    MCRT_HD void put_node(int k, uint32_t n) { node[k * stride + lane] = n; }
     MCRT_HD void push(int sp, uint32_t n, float l)
     {
         if (sp < depth)
@@ -275,6 +277,8 @@ struct Tracer
             // children as a mask over positions.  The nearest is visited
             // next; the others go on the stack farthest first, i.e. a child
             // lands above as many entries as passing children follow it.
+            // (Any-hit's boolean does not depend on the order, but this
+            // near-first order finds occluders sooner than slot order.)
             const uint32_t rk = ((r.oct & 4u) ? qb.w : qb.z) >> ((r.oct & 3u) * 8u);
             const uint32_t r0 = rk & 3u, r1 = (rk >> 2) & 3u, r2 = (rk >> 4) & 3u, r3 = (rk >> 6) & 3u;
             const uint32_t m = (p0 ? 1u << r0 : 0u) | (p1 ? 1u << r1 : 0u) | (p2 ? 1u << r2 : 0u) |
@@ -292,10 +296,20 @@ struct Tracer
                 {
                     // every push lands in LDS: four unconditional writes, those
                     // of slots not pushed into the discard row
-                    st.put(q0 ? i0 : st.depth, ch.x, l0);
-                    st.put(q1 ? i1 : st.depth, ch.y, l1);
-                    st.put(q2 ? i2 : st.depth, ch.z, l2);
-                    st.put(q3 ? i3 : st.depth, ch.w, l3);
+                    if (ANY)
+                    {
+                        st.put_node(q0 ? i0 : st.depth, ch.x);
+                        st.put_node(q1 ? i1 : st.depth, ch.y);
+                        st.put_node(q2 ? i2 : st.depth, ch.z);
+                        st.put_node(q3 ? i3 : st.depth, ch.w);
+                    }
+                    else
+                    {
+                        st.put(q0 ? i0 : st.depth, ch.x, l0);
+                        st.put(q1 ? i1 : st.depth, ch.y, l1);
+                        st.put(q2 ? i2 : st.depth, ch.z, l2);
+                        st.put(q3 ? i3 : st.depth, ch.w, l3);
+                    }
                 }
                 else
                 {
@@ -338,6 +352,21 @@ struct Tracer
                 cur = MCRT_CHILD_LEAF | ((count - 2u) << 29) | (first + 1u);
                 return false;
             }
+        }
+        if (ANY)
+        {
+            // tmax has not changed since the entry passed its test at push
+            // (it only shrinks at the hit that ends an any-hit traversal),
+            // so the entry is taken without the re-test (its LDS slot holds
+            // no distance)
+            if (sp == 0) return true;
+            --sp;
+            float lo;
+            if (sp < st.depth)
+                cur = st.node[sp * st.stride + st.lane];
+            else
+                st.pop(sp, cur, lo);
+            return false;
         }
         while (sp > 0)
         {

@@ -6,9 +6,12 @@
 // counter in device memory (`wq`, zero at launch): a lane whose ray ends
 // takes the next index of its wave's claim, and the wave claims again when
 // the claim is used up, so every lane stays busy until the batch runs out
-// instead of idling once its share of a static split is done.  The rays
-// are split into kSlices contiguous slices, one counter each (64 B apart),
-// and a wave starts on slice blockIdx % kSlices, i.e. on its XCD's slice as
+// instead of idling once its share of a static split is done.  Idle lanes
+// wait until at least S.refill_lanes of the wave's 64 lanes are idle (the
+// compaction threshold; 1 = refill every lane as soon as its ray ends),
+// so the ray loads of a refill can be gathered into fewer, fuller rounds.
+// The rays are split into kSlices contiguous slices, one counter each (64 B
+// apart), and a wave starts on slice blockIdx % kSlices, i.e. on its XCD's slice as
 // workgroups are dealt to the XCDs in turn; a wave whose slice is used up
 // moves on to the next.  Device-scope atomics on one address serialise, so
 // spreading them over eight counters and claiming kClaim rays (2 per lane)
@@ -93,10 +96,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
     // stalling the wave between short rays.
     uint32_t ahead = 0;  // lane 0: offset of the requested next window in slice sl
     if (lane == 0) ahead = atomicAdd(wq + sl * kSliceStride, kClaim);
+    const uint32_t refill = (uint32_t)S.refill_lanes;  // 1..64
     for (;;)
     {
-        // idle lanes take the next indices of the claim, in lane order
+        // idle lanes take the next indices of the claim, in lane order, once
+        // enough of them are idle (every lane idle always qualifies)
         uint64_t idle = __ballot(!live);
+        if ((uint32_t)__popcll(idle) < refill) idle = 0;
         while (idle != 0 && !drained)
         {
             if (claim == claim_end)

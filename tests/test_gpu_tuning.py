@@ -25,11 +25,13 @@ def _debug_scene(ctx):
 @pytest.mark.parametrize("waves", [1, 16, 40, 300])
 def test_traversal_tuning_bit_exact(ctx, depth, waves):
     import mcrt_amd
-    print(f"[tuning] depth={depth} waves={waves}", flush=True)
+    refill = {1: 64, 16: 1, 40: 7, 300: 16}[waves]  # compaction thresholds, one per row
+    print(f"[tuning] depth={depth} waves={waves} refill={refill}", flush=True)
     tris, s, st = _debug_scene(ctx)
     assert st["stack_bound"] > 16, "fixture spills at LDS depths 8, 12 and 16"
-    s.set_tuning(trace_waves_per_cu=waves, trace_lds_depth=depth)
-    assert s.tuning()["trace_lds_depth"] == depth and s.tuning()["trace_waves_per_cu"] == waves
+    s.set_tuning(trace_waves_per_cu=waves, trace_lds_depth=depth, trace_refill_lanes=refill)
+    t = s.tuning()
+    assert (t["trace_lds_depth"], t["trace_waves_per_cu"], t["trace_refill_lanes"]) == (depth, waves, refill)
     rays = _libs.random_rays(tris, 1 << 15, seed=11)
     prim, tuv = mcrt_amd.split_hits(s.intersect(rays))
     op, otuv = _libs.OracleBVH(tris).intersect(rays)
@@ -97,11 +99,13 @@ def test_render_invariant_under_tuning(ctx):
     s = d.to_scene(ctx, width=40, height=30)
     kw = dict(spp=8, max_depth=6, rr_start=2, rr_prob=0.8, seed=5)
     ref = s.render(**kw)
-    for waves, depth, chunk in ((4, 8, 1000), (32, 24, 1 << 20), (16, 12, 777), (1, 16, 5000)):
-        print(f"[render tuning] waves={waves} depth={depth} chunk={chunk}", flush=True)
-        s.set_tuning(pt_trace_waves_per_cu=waves, pt_trace_lds_depth=depth, pt_chunk_paths=chunk)
+    for waves, depth, chunk, refill in ((4, 8, 1000, 1), (32, 24, 1 << 20, 64), (16, 12, 777, 5), (1, 16, 5000, 32)):
+        print(f"[render tuning] waves={waves} depth={depth} chunk={chunk} refill={refill}", flush=True)
+        s.set_tuning(pt_trace_waves_per_cu=waves, pt_trace_lds_depth=depth, pt_chunk_paths=chunk,
+                     pt_trace_refill_lanes=refill)
         t = s.tuning()
-        assert (t["pt_trace_waves_per_cu"], t["pt_trace_lds_depth"], t["pt_chunk_paths"]) == (waves, depth, chunk)
+        assert (t["pt_trace_waves_per_cu"], t["pt_trace_lds_depth"], t["pt_chunk_paths"], t["pt_trace_refill_lanes"]) \
+            == (waves, depth, chunk, refill)
         f = s.render(**kw)
         assert np.array_equal(f.view(np.uint32), ref.view(np.uint32))
 
@@ -121,6 +125,7 @@ def test_autotune_applies_valid_tuning(ctx):
     assert t["trace_waves_per_cu"] in (8, 16, 24, 32, 64, 128)
     assert t["pt_chunk_paths"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
     assert t["pt_trace_waves_per_cu"] in (16, 32, 64) and t["pt_trace_lds_depth"] in (8, 16)
+    assert t["trace_refill_lanes"] in (1, 8, 16, 32) and t["pt_trace_refill_lanes"] in (1, 8, 16, 32)
     assert t["trace_reorder_rays"] in (-1, rays.shape[0])
     assert rate > 0
     assert np.array_equal(s.intersect(rays), before)
@@ -132,7 +137,7 @@ def test_tuning_errors(ctx):
     _, s, _ = _debug_scene(ctx)
     for kw in (dict(trace_lds_depth=10), dict(trace_waves_per_cu=1025), dict(trace_waves_per_cu=-1),
                dict(pt_chunk_paths=(1 << 28) + 1), dict(pt_trace_lds_depth=4), dict(pt_trace_waves_per_cu=2000),
-               dict(trace_reorder_rays=-2)):
+               dict(trace_reorder_rays=-2), dict(trace_refill_lanes=65), dict(pt_trace_refill_lanes=-1)):
         with pytest.raises(mcrt_amd.McrtError) as e:
             s.set_tuning(**kw)
         assert e.value.code == 1
