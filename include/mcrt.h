@@ -174,6 +174,14 @@ MCRT_API int mcrt_scene_set_camera(mcrt_scene* scene, const float o[3], const fl
 MCRT_API int mcrt_scene_finalize(mcrt_scene* scene, const mcrt_build_opts* opts,
                                  mcrt_build_stats* stats);
 MCRT_API int mcrt_scene_num_triangles(const mcrt_scene* scene, size_t* out);
+/* TriangleObject::setIntersectionEpsilon (src/core/triangle.cpp:53-56): the
+ * absolute determinant cut of the Moller-Trumbore test (a triangle is
+ * missed when |det| < eps; default 1e-6, NUM_EPS_DET).  The reference's
+ * value is process-global; here it belongs to the scene.  Allowed before or
+ * after finalize, but not while work on the scene is in flight; a batch
+ * keeps the value its scenes had at creation (mcrt_batch_render fails with
+ * MCRT_ERR_STATE once it changed). */
+MCRT_API int mcrt_scene_set_intersection_epsilon(mcrt_scene* scene, float eps);
 
 /* Ray queries.  mcrt_intersect = SplitBVH::intersect over each ray's own
  * [tmin, tmax] (SceneBVH::intersect uses tmin = 1e-4, tmax = +inf).
@@ -222,8 +230,13 @@ MCRT_API int mcrt_render(mcrt_scene* scene, const mcrt_render_params* params, fl
  * for that triangle).  A material shared by several triangles has the sum
  * of their entries as its gradient.  The same paths are replayed (the RNG is
  * counter-based), so memory does not grow with path length.  `film`
- * (optional, W*H*3) receives the primal film.  Gradients are not formed for
- * colour channels that are exactly 0. */
+ * (optional, W*H*3) receives the primal film.  At each path vertex the
+ * albedo gradient of channel c is formed as delta_c * (radiance gathered
+ * after the vertex) / albedo_c in fp32, so channels whose albedo is exactly
+ * 0 get no gradient (0, though the true derivative is not), and for small
+ * albedos the difference of the path's radiance sums loses about
+ * 6e-8 * L_path / (albedo * L_after) relative accuracy (1e-3 at albedo 1e-4
+ * when the radiance after the vertex is comparable to the path's). */
 MCRT_API int mcrt_render_adjoint(mcrt_scene* scene, const mcrt_render_params* params,
                                  const float* dL_dfilm, float* d_albedo, float* d_emission,
                                  float* film, int flags, void* stream);

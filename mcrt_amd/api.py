@@ -88,6 +88,7 @@ def lib():
         "mcrt_scene_set_camera": (I, [P, P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float]),
         "mcrt_scene_finalize": (I, [P, ctypes.POINTER(BuildOpts), ctypes.POINTER(BuildStats)]),
         "mcrt_scene_num_triangles": (I, [P, ctypes.POINTER(Z)]),
+        "mcrt_scene_set_intersection_epsilon": (I, [P, ctypes.c_float]),
         "mcrt_intersect": (I, [P, P, Z, P, I, P]),
         "mcrt_occluded": (I, [P, P, Z, P, I, P]),
         "mcrt_intersect_noalloc": (I, [P, P, Z, P, P]),
@@ -230,6 +231,10 @@ class Scene:
         _check(lib().mcrt_scene_load_obj(self.h, os.fsencode(path), None if m is None else m.ctypes.data, int(mat),
                                          ctypes.byref(n)))
         return n.value
+
+    def set_intersection_epsilon(self, eps):
+        """TriangleObject::setIntersectionEpsilon, per scene (default 1e-6)."""
+        _check(lib().mcrt_scene_set_intersection_epsilon(self.h, float(eps)))
 
     def set_camera(self, origin, target, width, height, fov_y):
         o = np.ascontiguousarray(origin, np.float32)

@@ -317,6 +317,16 @@ MCRT_API int mcrt_scene_set_camera(mcrt_scene* s, const float o[3], const float 
     return MCRT_OK;
 }
 
+MCRT_API int mcrt_scene_set_intersection_epsilon(mcrt_scene* s, float eps)
+{
+    if (!s) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_set_intersection_epsilon: NULL scene");
+    if (!(eps >= 0.0f && eps < __builtin_inff()))
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_set_intersection_epsilon: epsilon must be finite and >= 0");
+    s->det_eps = eps;
+    if (s->finalized) apply_trace_tuning(s);
+    return MCRT_OK;
+}
+
 MCRT_API int mcrt_scene_num_triangles(const mcrt_scene* s, size_t* out)
 {
     if (!s || !out) return fail(MCRT_ERR_INVALID_ARG, "mcrt_scene_num_triangles: NULL argument");
@@ -787,9 +797,14 @@ MCRT_API int mcrt_batch_render(mcrt_batch* b, void* stream)
     HIPCHK(hipSetDevice(b->ctx->device));
     hipStream_t st = (hipStream_t)stream;
     for (size_t i = 0; i < b->scenes.size(); i++)
+    {
+        if (b->scenes[i]->dev.det_eps != b->det_eps[i])
+            return fail(MCRT_ERR_STATE,
+                        "mcrt_batch_render: the intersection epsilon of scene %zu changed after mcrt_batch_create", i);
         if (b->scenes[i]->env_gen != b->env_gen[i])
             return fail(MCRT_ERR_STATE,
                         "mcrt_batch_render: the environment map of scene %zu changed after mcrt_batch_create", i);
+    }
     std::lock_guard<std::mutex> g(b->res.mu);
     if (!(b->flags & MCRT_BATCH_GRAPH)) return batch_enqueue(b, st);
     if (!b->exec)

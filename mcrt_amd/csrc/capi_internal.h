@@ -127,6 +127,8 @@ struct mcrt_scene
     // device residency
     mcrt::DevScene dev;
     mcrt::DevBuf d_nodes, d_refs, d_shade, d_verts, d_mats, d_lights, d_hot;
+    // Moller-Trumbore determinant cut (TriangleObject::setIntersectionEpsilon)
+    float det_eps = 1e-6f;
     // environment map (dev.env_*); env_gen counts replacements
     mcrt::DevBuf d_env_px, d_env_cdf_u, d_env_cdf_v;
     uint64_t env_gen = 0;
@@ -215,6 +217,7 @@ struct mcrt_batch
     int max_depth = 0;
     int rs = 2;                      // next-event rays per record (4: a scene has an environment map)
     std::vector<uint64_t> env_gen;   // each scene's environment generation at creation
+    std::vector<float> det_eps;      // each scene's intersection epsilon at creation
     int64_t total_paths = 0;
     std::vector<mcrt::BatchChunk> chunks;
     mcrt::DevBuf d_tab, d_roots, d_seg;

@@ -44,6 +44,8 @@ struct DevScene
     const int32_t* hot_tri = nullptr;  // triangle of each gradient hot slot
     int32_t nhot = 0;
     int32_t refill_lanes = 1;          // a traversal wave claims new rays once this many of its lanes are idle
+    float det_eps = 1e-6f;             // Moller-Trumbore |det| cut (TriangleObject::s_intersection_epsilon)
+    int32_t pad1 = 0;
     int64_t ntri = 0;
     uint64_t* spill = nullptr;         // traversal stack overflow storage
     int64_t spill_lanes = 0;           // lanes the spill buffer covers = resident lanes of a launch
@@ -80,6 +82,8 @@ struct SceneRoot
 {
     const QNode* nodes = nullptr;
     const WRef* refs = nullptr;
+    float det_eps = 1e-6f;  // the scene's Moller-Trumbore determinant cut
+    int32_t pad = 0;
 };
 
 // Work counters of a traversal launch: one per slice of the rays, 64 B

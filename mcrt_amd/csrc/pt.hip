@@ -1002,6 +1002,7 @@ int batch_init(mcrt_batch* b)
     b->max_depth = 0;
     b->rs = 2;
     b->env_gen.clear();
+    b->det_eps.clear();
     std::vector<ScenePT> tab(n);
     std::vector<SceneRoot> roots(n);
     for (size_t i = 0; i < n; i++)
@@ -1011,10 +1012,12 @@ int batch_init(mcrt_batch* b)
         tab[i].film = b->films[i];
         roots[i].nodes = s->dev.nodes;
         roots[i].refs = s->dev.refs;
+        roots[i].det_eps = s->dev.det_eps;
         pmax = std::max<int64_t>(pmax, b->params[i].spp);
         max_sb = std::max(max_sb, s->dev.stack_bound);
         if (s->dev.env_w > 0) b->rs = 4;
         b->env_gen.push_back(s->env_gen);
+        b->det_eps.push_back(s->dev.det_eps);
         b->max_depth = std::max(b->max_depth, b->params[i].max_depth);
     }
     // chunks of whole pixels, at most pmax paths each

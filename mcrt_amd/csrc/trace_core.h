@@ -120,15 +120,17 @@ MCRT_HD bool qslab_nf(const RayPrep& r, uint32_t child, uint32_t nx, uint32_t fx
     return (lo <= hi) & (lo <= R1) & (hi >= r.R0) & (child != MCRT_CHILD_EMPTY);
 }
 
-// rayTriangleIntersect (triangle.cpp:13-51), e1/e2 precomputed.
+// rayTriangleIntersect (triangle.cpp:13-51), e1/e2 precomputed; `eps` is
+// the absolute determinant cut (TriangleObject::s_intersection_epsilon,
+// 1e-6 unless set, triangle.cpp:9, :53-56).
 MCRT_HD bool mt(const RayPrep& r, float v0x, float v0y, float v0z, float e1x, float e1y,
-                                   float e1z, float e2x, float e2y, float e2z, float& t, float& u, float& v)
+                                   float e1z, float e2x, float e2y, float e2z, float eps, float& t, float& u, float& v)
 {
     float px = (r.dy * e2z) - (r.dz * e2y);
     float py = (r.dz * e2x) - (r.dx * e2z);
     float pz = (r.dx * e2y) - (r.dy * e2x);
     float det = e1x * px + e1y * py + e1z * pz;
-    if (fabsf(det) < 1e-6f) return false;
+    if (fabsf(det) < eps) return false;
     float inv = 1.0f / det;
     float tx = r.ox - v0x, ty = r.oy - v0y, tz = r.oz - v0z;
     u = (tx * px + ty * py + tz * pz) * inv;
@@ -217,7 +219,7 @@ struct Tracer
     }
 
     // The 64 B record `cur` names: a QNode, or a leaf reference.  Sc is any
-    // type with the scene's `nodes` and `refs` (DevScene, SceneRoot).
+    // type with the scene's `nodes`, `refs` and `det_eps` (DevScene, SceneRoot).
     template <class Sc>
     MCRT_HD const uint4* record(const Sc& S) const
     {
@@ -233,12 +235,12 @@ struct Tracer
     MCRT_HD bool step(const Sc& S, TraceStack& st, TraceCount& cnt)
     {
         const uint4* p = record(S);
-        return visit(p[0], p[1], p[2], p[3], st, cnt);
+        return visit(p[0], p[1], p[2], p[3], st, cnt, S.det_eps);
     }
 
-    // Visits `cur` given its record w0..w3.
+    // Visits `cur` given its record w0..w3 (eps: the scene's determinant cut).
     MCRT_HD bool visit(const uint4 w0, const uint4 w1, const uint4 w2, const uint4 w3, TraceStack& st,
-                       TraceCount& cnt)
+                       TraceCount& cnt, float eps)
     {
         if (!(cur & MCRT_CHILD_LEAF))
         {
@@ -338,7 +340,7 @@ struct Tracer
             {
                 float t, u, v;
                 if (COUNT) cnt.tris++;
-                if (mt(r, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, t, u, v) && t > r.R0 && t < R1)
+                if (mt(r, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, eps, t, u, v) && t > r.R0 && t < R1)
                 {
                     R1 = t;
                     bu = u;

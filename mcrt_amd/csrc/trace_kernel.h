@@ -82,6 +82,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
     SceneRoot root;
     root.nodes = S.nodes;
     root.refs = S.refs;
+    root.det_eps = S.det_eps;
     uint32_t i = 0;
     const uint32_t* order = ORDER && *B.skip == 0 ? B.order : nullptr;
     uint32_t claim = 0, claim_end = 0;  // wave-uniform: indices [claim, claim_end) not yet started

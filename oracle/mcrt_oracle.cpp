@@ -35,7 +35,9 @@ namespace orc
 {
 static const float kInf = std::numeric_limits<float>::infinity();
 static const float kPi = 3.14159265358979323846f;  // constants.cpp:11,19
-static const float kEpsDet = 1e-6f;                 // constants.cpp:15
+// TriangleObject::s_intersection_epsilon (triangle.cpp:9): NUM_EPS_DET
+// (constants.cpp:15) unless set, process-global as in the reference.
+static float kEpsDet = 1e-6f;
 static const float kEpsRay = 1e-4f;                 // constants.cpp:17
 static const float kEpsCos = 1e-6f;                 // constants.cpp:19
 static const int kLeafSize = 4;                     // splitbvh.h:27
@@ -838,6 +840,9 @@ ORC_API long long orc_load_obj(const char* path, const float* m16, float* tris, 
         }
     return nf;
 }
+
+// TriangleObject::setIntersectionEpsilon (triangle.cpp:53-56).
+ORC_API void orc_set_intersection_epsilon(float eps) { kEpsDet = eps; }
 
 ORC_API int orc_num_procs()
 {

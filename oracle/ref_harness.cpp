@@ -308,6 +308,9 @@ REF_API void ref_shading_normal(const float* tris9, const float* vn9, const floa
     }
 }
 
+// TriangleObject::setIntersectionEpsilon (process-global in the reference).
+REF_API void ref_set_intersection_epsilon(float eps) { TriangleObject::setIntersectionEpsilon(eps); }
+
 // Camera::makeRay at (mx, my) = pxy / (w, h); rays6 n x 6.
 REF_API void ref_camera_rays(const float* o, const float* at, int w, int h, float fov,
                              const float* pxy, long long n, float* rays6)

@@ -62,6 +62,7 @@ def oracle():
         "orc_shading_normal": (None, [P, P, P, LL, P]),
         "orc_tent": (None, [P, LL, P]),
         "orc_pixel_jitter": (None, [U64, P, P, LL, P]),
+        "orc_set_intersection_epsilon": (None, [F]),
     })
 
 
@@ -79,6 +80,7 @@ def reference():
         "ref_load_obj": (LL, [ctypes.c_char_p, P, P, P]),
         "ref_demo_transforms": (None, [P]),
         "ref_shading_normal": (None, [P, P, P, LL, P]),
+        "ref_set_intersection_epsilon": (None, [F]),
         "ref_camera_rays": (None, [P, P, I, I, F, P, LL, P]),
         "ref_bsdf": (None, [I, P, P, LL, P]),
         "ref_discrete": (None, [P, I, P, LL, P, P]),

@@ -339,8 +339,43 @@ def shading_normal_fixture(R):
     print("shading normal", t.shape[0])
 
 
+EPSILONS = (0.0, 1e-7, 1e-5)
+
+
+def eps_fixture(R):
+    """SplitBVH::intersect with TriangleObject::setIntersectionEpsilon
+    (triangle.cpp:53-56) at 0, 1e-7 and 1e-5 (default 1e-6) on the
+    DebugMesh and the soup slice, whose small triangles sit near the cut.
+    Stored per epsilon: SHA-256 of the prim and t/u/v arrays, and in full
+    the rays whose result differs from the default cut's (bvh_<name>.npz)."""
+    out = {}
+    try:
+        for name, (tris, rays) in scene_inputs().items():
+            if name == "cornell_ico4":
+                continue
+            h, d, st = ref_bvh(R, tris)
+            R.ref_set_intersection_epsilon(1e-6)
+            p0, t0 = ref_hits(R, h, rays)
+            for e in EPSILONS:
+                R.ref_set_intersection_epsilon(e)
+                prim, tuv = ref_hits(R, h, rays)
+                diff = np.nonzero((prim != p0) | np.any(tuv.view(np.uint32) != t0.view(np.uint32), axis=1))[0]
+                key = f"{name}_{e:g}"
+                out[key + "_sha"] = sha(prim, tuv)
+                out[key + "_idx"] = diff.astype(np.int32)
+                out[key + "_prim"] = prim[diff]
+                out[key + "_tuv"] = tuv[diff]
+                print(name, e, "hit", (prim >= 0).mean(), "differ from 1e-6:", diff.size)
+            R.ref_bvh_free(h)
+    finally:
+        R.ref_set_intersection_epsilon(1e-6)
+    np.savez_compressed(os.path.join(HERE, "eps.npz"), **out)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["obj_models"]:
+    if sys.argv[1:] == ["eps"]:
+        eps_fixture(_libs.reference())
+    elif sys.argv[1:] == ["obj_models"]:
         obj_models_fixture(_libs.reference())
     elif sys.argv[1:] == ["shading_normal"]:
         shading_normal_fixture(_libs.reference())
@@ -356,3 +391,4 @@ if __name__ == "__main__":
         envmap_fixture(_libs.reference())
         obj_models_fixture(_libs.reference())
         shading_normal_fixture(_libs.reference())
+        eps_fixture(_libs.reference())

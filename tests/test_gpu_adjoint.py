@@ -36,6 +36,35 @@ def test_adjoint_matches_oracle_cpu_adjoint(ctx):
     assert np.all(ga[:2] == 0) and np.all(ge[2:] == 0)
 
 
+def test_adjoint_small_and_zero_albedo(ctx):
+    """A vertex's albedo gradient is formed as delta * (L_total - L_before) /
+    albedo, channel by channel.  Small albedos (1e-3, 1e-4) follow the
+    oracle's adjoint on the same paths as closely as ordinary ones; a
+    channel whose albedo is exactly 0 gets no gradient (0), although its
+    true derivative is not 0 (documented at mcrt_render_adjoint)."""
+    d = _small()
+    rgb = d.mat_rgb.copy()
+    sphere = np.arange(4, rgb.shape[0])  # materials 0..3: light, grey, red, green walls
+    rgb[sphere[0::3]] = [1e-3, 0.5, 1e-4]
+    rgb[sphere[1::3], 1] = 0.0
+    d.mat_rgb[:] = rgb
+    w, h = 24, 20
+    s = d.to_scene(ctx, width=w, height=h)
+    dfilm = np.random.default_rng(4).uniform(-1, 1, (h, w, 3)).astype(np.float32)
+    ga, ge = s.render_adjoint(dfilm, **KW)
+    oa, oe, _ = _libs.OracleScene(d).adjoint(dfilm, w, h, precision=0, **KW)
+    zero = np.isin(d.tri_mat, sphere[1::3])
+    assert zero.sum() > 10 and np.all(ga[zero, 1] == 0)
+    small = np.isin(d.tri_mat, sphere[0::3])
+    for c in (0, 2):  # the 1e-3 and 1e-4 channels
+        g, o = ga[small, c], oa[small, c]
+        assert np.abs(o).max() > 0
+        assert np.max(np.abs(g - o)) <= 2e-4 * np.abs(oa).max()
+        big = np.abs(o) > 1e-2 * np.abs(o).max()
+        assert big.sum() > 5
+        assert np.max(np.abs(g[big] - o[big]) / np.abs(o[big])) < 1e-3
+
+
 def test_device_buffers_and_repeatability(ctx):
     import torch
     d = _small()

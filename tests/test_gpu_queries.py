@@ -63,6 +63,49 @@ def test_soup_slice(ctx):
     _check_closest(ctx, tris, rays, "soup_slice")
 
 
+def test_intersection_epsilon_vs_reference(ctx):
+    """mcrt_scene_set_intersection_epsilon = TriangleObject::
+    setIntersectionEpsilon (triangle.cpp:53-56): on the soup slice the hits
+    at cuts 0, 1e-7 and 1e-5 equal the compiled reference's (eps.npz), set
+    before and after finalize; a batch refuses to render once a scene's cut
+    changed; bad values are refused."""
+    import hashlib
+    import os
+    import torch
+    import mcrt_amd
+    import make_golden
+    from mcrt_amd import scenes
+    tris, rays = make_golden.scene_inputs()["soup_slice"]
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "eps.npz"))
+    s = mcrt_amd.Scene(ctx)
+    s.add(tris, s.add_material(mcrt_amd.DIFF, (0.5, 0.5, 0.5)))
+    s.set_intersection_epsilon(1e-5)  # before finalize
+    s.finalize()
+    for e in (1e-5, 0.0, 1e-7):
+        s.set_intersection_epsilon(e)
+        prim, tuv = mcrt_amd.split_hits(s.intersect(rays))
+        h = hashlib.sha256()
+        h.update(np.ascontiguousarray(prim).tobytes())
+        h.update(np.ascontiguousarray(tuv).tobytes())
+        assert h.hexdigest() == str(g[f"soup_slice_{e:g}_sha"]), e
+    for bad in (-1.0, float("inf"), float("nan")):
+        with pytest.raises(mcrt_amd.McrtError) as e:
+            s.set_intersection_epsilon(bad)
+        assert e.value.code == 1
+    s.set_intersection_epsilon(1e-6)
+    d = scenes.cornell_icosphere(1)
+    sc = d.to_scene(ctx, width=8, height=8)
+    film = torch.zeros((8, 8, 3), dtype=torch.float32, device="cuda")
+    b = mcrt_amd.Batch([sc], [dict(spp=1, max_depth=2)], [film])
+    b.render()
+    torch.cuda.synchronize()
+    sc.set_intersection_epsilon(1e-5)
+    with pytest.raises(mcrt_amd.McrtError) as e:
+        b.render()
+    assert e.value.code == 2
+    b.close()
+
+
 def test_axis_aligned_rays_miss_like_reference(ctx):
     # A zero direction component gives inv_dir = 0 and the slab test rejects
     # every box (aabb.cpp:128-191 with ray.cpp:77-79): such rays miss.

@@ -178,3 +178,6 @@ def test_render_errors(ctx):
     with pytest.raises(mcrt_amd.McrtError) as e:
         s.render(spp=1, rr_prob=0.0)
     assert e.value.code == 1
+    with pytest.raises(mcrt_amd.McrtError) as e:
+        s.render(spp=(1 << 28) + 1)  # one pixel's samples must fit a wavefront chunk
+    assert e.value.code == 1 and b"spp" in mcrt_amd.lib().mcrt_last_error()

@@ -153,6 +153,33 @@ def test_builder_and_traversal_vs_reference_fixture(orc, name):
     assert np.array_equal(bits(tuv), bits(g["tuv"]))
 
 
+@pytest.mark.parametrize("name", ["debug_mesh", "soup_slice"])
+def test_intersection_epsilon_vs_reference_fixture(orc, name):
+    """TriangleObject::setIntersectionEpsilon (triangle.cpp:53-56): the
+    traversal's hits with the determinant cut at 0, 1e-7 and 1e-5 equal the
+    compiled reference's (eps.npz holds their SHA-256 and the rays whose
+    result differs from the default cut's); the soup slice's small
+    triangles make the cut matter."""
+    import make_golden
+    tris, rays = make_golden.scene_inputs()[name]
+    g = gold("eps.npz")
+    base = gold(f"bvh_{name}.npz")
+    ob = _libs.OracleBVH(tris)
+    try:
+        for e in make_golden.EPSILONS:
+            orc.orc_set_intersection_epsilon(e)
+            prim, tuv = ob.intersect(rays)
+            key = f"{name}_{e:g}"
+            assert sha(prim, tuv) == str(g[key + "_sha"]), key
+            idx = g[key + "_idx"]
+            assert np.array_equal(prim[idx], g[key + "_prim"]) and np.array_equal(bits(tuv[idx]), bits(g[key + "_tuv"]))
+            same = np.setdiff1d(np.arange(rays.shape[0]), idx)
+            assert np.array_equal(prim[same], base["prim"][same])
+    finally:
+        orc.orc_set_intersection_epsilon(1e-6)
+    assert g["soup_slice_1e-05_idx"].size > 1000 and g["soup_slice_0_idx"].size > 10
+
+
 def test_builder_vs_live_reference(orc, ref):
     rng = np.random.default_rng(17)
     for n in (5, 37, 300, 3000):
