@@ -139,11 +139,11 @@ struct mcrt_scene
     // defaults measured on MI355X (1.3M-triangle displaced icosphere, 16M
     // incoherent rays; 1024^2 renders of it and of an outdoor scene): waves
     // claim rays from work counters, so the grid only has to fill the
-    // machine once: 32 one-wave workgroups per CU with 8-entry LDS stacks
-    // (8 waves per SIMD) were fastest for queries and renders alike
-    int32_t trace_waves_per_cu = 32;
+    // machine once: 28 one-wave workgroups per CU with 8-entry LDS stacks
+    // (7 waves per SIMD, all resident) were fastest for queries and renders alike
+    int32_t trace_waves_per_cu = 28;
     int32_t trace_lds_depth = 8;
-    int32_t pt_trace_waves_per_cu = 32;
+    int32_t pt_trace_waves_per_cu = 28;
     int32_t pt_trace_lds_depth = 8;
     // 2^26 paths per chunk (a whole 1024^2 x 64-spp frame; about 30 GB of
     // path state, 45 GB with an environment map): fewer, larger launches

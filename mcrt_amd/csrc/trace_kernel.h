@@ -44,11 +44,13 @@ struct TraceBatch
     const uint32_t* skip = nullptr;
 };
 
-// With an 8-entry stack, eight waves per SIMD (32 per CU) fit in LDS; they
-// also need at most 64 VGPRs and 96 SGPRs, which the attribute asks for.
+// With an 8-entry stack the kernel is compiled for seven waves per SIMD (28
+// per CU; 66 VGPRs, and LDS for all of them).  The register budget of eight
+// waves (64 VGPRs) made the compiler spill 24 SGPRs into VGPR lanes, and the
+// spill code in the refill block cost more than the eighth wave gained.
 // Deeper stacks are held to fewer waves by LDS.
 template <int D, bool ANY, bool COUNT, bool SEG, bool BATCH, bool ORDER>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 8 : 4, 8))) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 7 : 4, 8))) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
                                                  const uint32_t* __restrict__ seg_cnt, int64_t seg1,
                                                  int4* __restrict__ hits, uint8_t* __restrict__ occ,
                                                  unsigned long long* __restrict__ counters, TraceBatch B,

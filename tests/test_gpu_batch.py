@@ -103,6 +103,37 @@ def test_batch_graph_replays(ctx):
     b.close()
 
 
+def test_batch_autotune(ctx):
+    """mcrt_batch_autotune picks its settings from the documented lists,
+    applies them to the first scene, drops the captured graph, and the films
+    rendered after it are still those of each scene rendered alone."""
+    import torch
+    import mcrt_amd
+    sc = _scenes(ctx)
+    ref = [s.render(**p) for s, p in sc]
+    dev = [torch.full((s.camera[3], s.camera[2], 3), -1.0, dtype=torch.float32, device="cuda") for s, _ in sc]
+    b = mcrt_amd.Batch([s for s, _ in sc], [p for _, p in sc], dev, graph=True)
+    b.render()  # captures the graph
+    torch.cuda.synchronize()
+    t, ms = b.autotune()
+    print(f"[batch autotune] {t} {ms:.3f} ms", flush=True)
+    assert ms > 0
+    assert t["pt_trace_waves_per_cu"] in (16, 28, 64) and t["pt_trace_lds_depth"] in (8, 16)
+    assert t["pt_trace_refill_lanes"] in (1, 8, 16, 32)
+    assert t["pt_chunk_paths"] in [1 << k for k in range(22, 27)]
+    got = sc[0][0].tuning()
+    for k in ("pt_trace_waves_per_cu", "pt_trace_lds_depth", "pt_trace_refill_lanes", "pt_chunk_paths"):
+        assert got[k] == t[k], k
+    for it in range(2):
+        for f in dev:
+            f.fill_(-1.0)
+        b.render()  # the first captures the graph again, the second replays it
+        torch.cuda.synchronize()
+        for r, f in zip(ref, dev):
+            assert np.array_equal(r.view(np.uint32), f.cpu().numpy().view(np.uint32)), f"render {it}"
+    b.close()
+
+
 def test_batch_errors(ctx):
     import mcrt_amd
     sc = _scenes(ctx)[:2]

@@ -122,9 +122,9 @@ def test_autotune_applies_valid_tuning(ctx):
     print(f"[autotune] {t} {rate:.1f} Mrays/s", flush=True)
     assert t == s.tuning()
     assert t["trace_lds_depth"] in (8, 12, 16)
-    assert t["trace_waves_per_cu"] in (8, 16, 24, 32, 64, 128)
+    assert t["trace_waves_per_cu"] in (8, 16, 24, 28, 64, 128)
     assert t["pt_chunk_paths"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
-    assert t["pt_trace_waves_per_cu"] in (16, 32, 64) and t["pt_trace_lds_depth"] in (8, 16)
+    assert t["pt_trace_waves_per_cu"] in (16, 28, 64) and t["pt_trace_lds_depth"] in (8, 16)
     assert t["trace_refill_lanes"] in (1, 8, 16, 32) and t["pt_trace_refill_lanes"] in (1, 8, 16, 32)
     assert t["trace_reorder_rays"] in (-1, rays.shape[0])
     assert rate > 0
