@@ -82,8 +82,11 @@ struct PTState
 };
 
 // float4 words of a next-event record: {T, pdf_b}, {albedo, factor2},
-// {wb, path}, {wo, light}, and with the environment {e1}, {e2}.
-__host__ __device__ constexpr int rec_words(int rs) { return rs == 4 ? 6 : 4; }
+// {path, light, -, -}; with the environment {T, pdf_b}, {albedo, factor2},
+// {e1, path}, {e2, light}.  The directions of its rays are read back from
+// the rays themselves (the resolve step needs them only where a ray hit an
+// emitter).
+__host__ __device__ constexpr int rec_words(int rs) { return rs == 4 ? 4 : 3; }
 
 // One scene of a render: its device data, camera and parameters.
 struct ScenePT
@@ -344,9 +347,11 @@ __global__ void k_generate(PTState S, PTArgs a)
 // then the light sample if its shadow ray reaches the sampled light.
 // em1/em2, hot1/hot2 and the coefficients c1/c2 of the two emissions are
 // what the adjoint scatters.
-__device__ __forceinline__ V3f area_terms(const DevScene& D, const int4 h1, const int4 h2, V3f col, V3f wb, V3f wo,
-                                          float pdf_b, float factor2, int li, V3f local, int& em1, int& em2,
-                                          int& hot1, int& hot2, V3f& c1, V3f& c2)
+// nr: the record's rays (BSDF-sampled ray, then the light sample's shadow
+// ray; 2 float4 each, the direction in the second).
+__device__ __forceinline__ V3f area_terms(const DevScene& D, const int4 h1, const int4 h2, V3f col,
+                                          const float4* nr, float pdf_b, float factor2, int li, V3f local, int& em1,
+                                          int& em2, int& hot1, int& hot2, V3f& c1, V3f& c2)
 {
     float pdf_l = 0.0f;
     V3f le1 = v3f(0.0f, 0.0f, 0.0f);
@@ -357,6 +362,7 @@ __device__ __forceinline__ V3f area_terms(const DevScene& D, const int4 h1, cons
         if (mt.type == MCRT_LGHT)
         {
             const V3f ns1 = shading_normal(ts, float_of((uint32_t)h1.z), float_of((uint32_t)h1.w));
+            const V3f wb = f3(nr[1]);
             const float cosl = dot(ns1, -wb);
             if (!(cosl < 1e-6f))
             {
@@ -387,6 +393,7 @@ __device__ __forceinline__ V3f area_terms(const DevScene& D, const int4 h1, cons
             if (mt.type == MCRT_LGHT)
             {
                 const V3f ns2 = shading_normal(ts, float_of((uint32_t)h2.z), float_of((uint32_t)h2.w));
+                const V3f wo = f3(nr[3]);
                 if (!(dot(ns2, -wo) < 0) && ts.light == li)
                 {
                     le2 = ld3(mt.rgb);
@@ -425,25 +432,25 @@ __global__ void k_resolve(PTArgs a, PTState S, int cur)
         if (act)
         {
             const float4* Rm = R + rec_words(rs) * m;
-            const float4 r0 = Rm[0], r1 = Rm[1], r2 = Rm[2], r3 = Rm[3];
+            const float4 r0 = Rm[0], r1 = Rm[1], r2 = Rm[2], r3 = ENV ? Rm[3] : r2;
             const V3f T = f3(r0), col = f3(r1);
-            const uint32_t path = bits_of(r2.w);
+            const uint32_t path = bits_of(ENV ? r2.w : r2.x);
             const DevScene& D = BATCH ? a.tab[S.sid[path]].D : a.one.D;
             const int64_t h0 = P + rs * m;
             V3f local = v3f(0.0f, 0.0f, 0.0f);
             if (ENV)
             {
                 const V3f z = v3f(0.0f, 0.0f, 0.0f);
-                const V3f e1 = S.hits[h0 + 2].x < 0 ? f3(Rm[4]) : z;
-                const V3f e2 = S.hits[h0 + 3].x < 0 ? f3(Rm[5]) : z;
+                const V3f e1 = S.hits[h0 + 2].x < 0 ? f3(r2) : z;
+                const V3f e2 = S.hits[h0 + 3].x < 0 ? f3(r3) : z;
                 local = local + (e1 + e2);
             }
             if (D.nlights > 0)
             {
-                const int li = (int)bits_of(r3.w);
+                const int li = (int)bits_of(ENV ? r3.w : r2.y);
                 V3f c1, c2;
-                local = area_terms(D, S.hits[h0], li >= 0 ? S.hits[h0 + 1] : make_int4(-1, 0, 0, 0), col, f3(r2),
-                                   f3(r3), r0.w, r1.w, li, local, em1, em2, hot1, hot2, c1, c2);
+                local = area_terms(D, S.hits[h0], li >= 0 ? S.hits[h0 + 1] : make_int4(-1, 0, 0, 0), col,
+                                   S.rays[cur] + 2 * h0, r0.w, r1.w, li, local, em1, em2, hot1, hot2, c1, c2);
                 if (ADJ)
                 {
                     const V3f dl = f3(S.delta[path]);
@@ -652,8 +659,7 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 }
                 R[0] = make_float4(T.x, T.y, T.z, pdf_b);
                 R[1] = make_float4(col.x, col.y, col.z, factor2);
-                R[2] = make_float4(wb.x, wb.y, wb.z, float_of(p));
-                R[3] = make_float4(wo.x, wo.y, wo.z, float_of((uint32_t)li_rec));
+                if (!ENV) R[2] = make_float4(float_of(p), float_of((uint32_t)li_rec), 0.0f, 0.0f);
                 // The BSDF-sampled ray adds a term only when its closest hit is
                 // an emitter, and the traversal accepts a triangle only after its
                 // reference's box passes the slab test.  That test is monotone in
@@ -678,8 +684,8 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 {
                     V3f e1 = zero, e2 = zero, d1 = zero, d2 = zero;
                     if (D.env_w > 0) env_terms(D, key, ns, col, e1, e2, d1, d2);
-                    R[4] = f4(e1);
-                    R[5] = f4(e2);
+                    R[2] = make_float4(e1.x, e1.y, e1.z, float_of(p));
+                    R[3] = make_float4(e2.x, e2.y, e2.z, float_of((uint32_t)li_rec));
                     store_ray(nr, r0i + 2, phit, d1, kEpsRay, __builtin_inff());
                     store_ray(nr, r0i + 3, phit, d2, kEpsRay, __builtin_inff());
                 }
