@@ -44,10 +44,11 @@ struct TraceBatch
     const uint32_t* skip = nullptr;
 };
 
-// With an 8-entry stack the kernel is compiled for seven waves per SIMD (28
-// per CU; 66 VGPRs, and LDS for all of them).  The register budget of eight
-// waves (64 VGPRs) made the compiler spill 24 SGPRs into VGPR lanes, and the
-// spill code in the refill block cost more than the eighth wave gained.
+// With an 8-entry stack the kernel is compiled for at least seven waves per
+// SIMD (28 per CU, with LDS for all of them; the closest-hit kernel takes 57
+// VGPRs).  Held to eight waves' budget the compiler spilled SGPRs into VGPR
+// lanes, and the spill code in the refill block cost more than the eighth
+// wave gained.
 // Deeper stacks are held to fewer waves by LDS.
 template <int D, bool ANY, bool COUNT, bool SEG, bool BATCH, bool ORDER>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 7 : 4, 8))) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
@@ -96,7 +97,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
     // The next window is requested when the current one is taken, and its
     // offset is read (lane 0's atomic result) only when that one runs out,
     // so the atomic's round trip overlaps traversal steps instead of
-    // stalling the wave between short rays.
+    // stalling the wave between short rays.  Every lane runs the refill loop
+    // (its conditions are wave-uniform), so lane 0 is the first active lane
+    // there and readfirstlane hands its result to the wave as a scalar: the
+    // claim bookkeeping stays in scalar registers instead of being shuffled
+    // through the LDS crossbar into vector registers.
     uint32_t ahead = 0;  // lane 0: offset of the requested next window in slice sl
     if (lane == 0) ahead = atomicAdd(wq + sl * kSliceStride, kClaim);
     const uint32_t refill = (uint32_t)S.refill_lanes;  // 1..64
@@ -110,7 +115,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
         {
             if (claim == claim_end)
             {
-                uint32_t c = __shfl(ahead, 0);
+                uint32_t c = __builtin_amdgcn_readfirstlane(ahead);
                 uint32_t lo = slice_lo(sl), len = slice_lo(sl + 1) - lo;
                 while (c >= len)  // slice used up: move on to the next one
                 {
@@ -124,7 +129,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
                     len = slice_lo(sl + 1) - lo;
                     uint32_t t = 0;
                     if (lane == 0) t = atomicAdd(wq + sl * kSliceStride, kClaim);
-                    c = __shfl(t, 0);
+                    c = __builtin_amdgcn_readfirstlane(t);
                 }
                 if (drained) break;
                 if (lane == 0) ahead = atomicAdd(wq + sl * kSliceStride, kClaim);
