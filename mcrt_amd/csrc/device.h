@@ -45,7 +45,7 @@ struct DevScene
     int32_t nhot = 0;
     int32_t refill_lanes = 1;          // a traversal wave claims new rays once this many of its lanes are idle
     float det_eps = 1e-6f;             // Moller-Trumbore |det| cut (TriangleObject::s_intersection_epsilon)
-    int32_t pad1 = 0;
+    float node_span = 0.0f;            // qnode_span of the nodes (wide.h): scale of the node-test margin
     int64_t ntri = 0;
     uint64_t* spill = nullptr;         // traversal stack overflow storage
     int64_t spill_lanes = 0;           // lanes the spill buffer covers = resident lanes of a launch
@@ -83,7 +83,7 @@ struct SceneRoot
     const QNode* nodes = nullptr;
     const WRef* refs = nullptr;
     float det_eps = 1e-6f;  // the scene's Moller-Trumbore determinant cut
-    int32_t pad = 0;
+    float node_span = 0.0f; // qnode_span of the scene's nodes
 };
 
 // Work counters of a traversal launch: one per slice of the rays, 64 B

@@ -78,7 +78,10 @@ inline uint32_t order_to_rank(uint32_t order)
 }
 static_assert(sizeof(QNode) == 64, "QNode must be 64 bytes");
 
-// Decoded plane of a QNode (the kernels evaluate the same expression).
+// Decoded plane of a QNode: the host checks with it that every decoded box
+// contains the exact child box.  The kernels test the decoded box
+// conservatively (trace_core.h: node_margin), passing wherever the
+// reference's slab arithmetic on this plane passes.
 inline float qnode_plane(float origin, uint32_t q, float scale) { return origin + (float)q * scale; }
 
 struct alignas(64) WRef

@@ -1019,6 +1019,7 @@ int batch_init(mcrt_batch* b)
         roots[i].nodes = s->dev.nodes;
         roots[i].refs = s->dev.refs;
         roots[i].det_eps = s->dev.det_eps;
+        roots[i].node_span = s->dev.node_span;
         pmax = std::max<int64_t>(pmax, b->params[i].spp);
         max_sb = std::max(max_sb, s->dev.stack_bound);
         if (s->dev.env_w > 0) b->rs = 4;

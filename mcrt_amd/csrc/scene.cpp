@@ -437,6 +437,7 @@ int finalize_scene(mcrt_scene* s, int threads, const char* cache)
     d.hot_tri = s->d_hot.as<int32_t>();
     d.nhot = (int32_t)hot_tri.size();
     d.stack_bound = wb.stack_bound;
+    d.node_span = qnode_span(wb.qnodes);
     d.ntri = (int64_t)n;
     d.emit_any = emit_any ? 1 : 0;
     for (int k = 0; k < 3; k++)

@@ -37,9 +37,12 @@ struct RayPrep
     float ox, oy, oz, dx, dy, dz;
     float ix, iy, iz;
     float R0;
+    float w;     // node-test margin scale (node_margin)
     bool sx, sy, sz;
-    bool exact;  // an inv_dir component is infinite: use the reference's
-                 // sequential test with its NaN behaviour
+    bool exact;  // an inv_dir component is infinite (or the node-test
+                 // margins would leave their bounds): use the reference's
+                 // sequential test with its NaN behaviour and descend every
+                 // child of a node
     uint32_t oct;
 };
 
@@ -54,6 +57,7 @@ MCRT_HD RayPrep prep_ray(float ox, float oy, float oz, float dx, float dy, float
     r.ix = inv_or_zero(dx); r.iy = inv_or_zero(dy); r.iz = inv_or_zero(dz);
     r.sx = dx < 0.0f; r.sy = dy < 0.0f; r.sz = dz < 0.0f;
     r.R0 = tmin;
+    r.w = 0.0f;
     r.exact = __builtin_isinf(r.ix) || __builtin_isinf(r.iy) || __builtin_isinf(r.iz);
     r.oct = (dx > 0.0f ? 1u : 0u) | (dy > 0.0f ? 2u : 0u) | (dz > 0.0f ? 4u : 0u);
     return r;
@@ -92,29 +96,54 @@ MCRT_HD bool slab(const RayPrep& r, float lx, float hx, float ly, float hy, floa
     return true;
 }
 
-// A QNode plane (layout.h: qnode_plane): origin + q * 2^k, q one byte.
-// The product q * 2^k is exact (q < 256 needs 8 significant bits, and a
-// normal 2^k keeps it normal), so the fused multiply-add rounds once, in
-// the addition, and equals layout.h's origin + q * scale bit for bit in one
-// instruction instead of two.
-MCRT_HD float qplane(float origin, uint32_t word, int shift, float scale)
+// Node tests are conservative.  A node test only decides which subtrees are
+// opened, and opening one more changes no result: its references' exact
+// clipped boxes then fail (the argument that makes the quantised boxes
+// exact, layout.h: QNode).  So a node test may use cheaper arithmetic as long
+// as it passes wherever the reference's slab arithmetic on the decoded box,
+// t = RN(RN(plane - o) * inv), passes.  With plane = RN(origin + q * s) it
+// maps the plane code q straight to
+//     t_near = fma(q, s * inv, (origin - o) * inv - w * |inv|)
+//     t_far  = fma(q, s * inv, (origin - o) * inv + w * |inv|),
+// two instructions per plane instead of four (s * inv is exact: s = 2^k).
+// Against the reference arithmetic its rounding error is below
+// u * |inv| * (5 |origin - o| + |plane| + 510 s) (u = 2^-24), plus
+// subnormal terms below 2^-140.  The margin w * |inv| covers both when
+// w = 2^-21 (max |o| + 2 span), span >= |origin| + 255 s of every node
+// (qnode_span), and w * |inv| >= 2^-100 for every non-zero inv.  So t_near
+// is at most, and t_far at least, the reference value, and the node test
+// passes wherever the reference-arithmetic test passes.  With inv = 0 both
+// are exactly the reference's 0.  Every quantity stays finite while
+// w * |inv| < 2^100.  A ray outside these bounds is marked exact: it
+// descends every child and is decided by the exact reference tests.
+MCRT_HD void node_margin(RayPrep& r, float span)
 {
-    return __builtin_fmaf((float)((word >> shift) & 0xffu), scale, origin);
+    const float om = fmaxf(fmaxf(fabsf(r.ox), fabsf(r.oy)), fabsf(r.oz));
+    const float w = fmaxf(0x1p-21f * __builtin_fmaf(2.0f, span, om), 0x1p-60f);
+    const float px = w * fabsf(r.ix), py = w * fabsf(r.iy), pz = w * fabsf(r.iz);
+    const bool okx = r.ix == 0.0f || (px >= 0x1p-100f && px < 0x1p100f);
+    const bool oky = r.iy == 0.0f || (py >= 0x1p-100f && py < 0x1p100f);
+    const bool okz = r.iz == 0.0f || (pz >= 0x1p-100f && pz < 0x1p100f);
+    r.w = w;
+    r.exact = r.exact || !(okx && oky && okz);
 }
 
-// Slab test of child `shift / 8` of a QNode against its decoded
-// (outward-rounded) box, given the plane words already ordered near / far
-// per axis by the ray's signs: the same operations, in the same order, as
-// slab() on the decoded box.
-MCRT_HD bool qslab_nf(const RayPrep& r, uint32_t child, uint32_t nx, uint32_t fx, uint32_t ny, uint32_t fy,
-                      uint32_t nz, uint32_t fz, float ox, float oy, float oz, float sx, float sy, float sz, int shift,
-                      float R1, float& lo)
+// Conservative test of child `shift / 8` of a QNode (node_margin), given the
+// plane words already ordered near / far per axis by the ray's signs, and
+// per axis the plane map's slope s * inv (a*) and its lower / upper offsets
+// (n*, f*).
+MCRT_HD bool qslab_c(const RayPrep& r, uint32_t child, uint32_t nx, uint32_t fx, uint32_t ny, uint32_t fy,
+                     uint32_t nz, uint32_t fz, float ax, float ay, float az, float nbx, float fbx, float nby,
+                     float fby, float nbz, float fbz, int shift, float R1, float& lo)
 {
     // computed for every slot (an empty slot's planes decode to finite
     // values), so the four tests need no branches
-    const float t0x = (qplane(ox, nx, shift, sx) - r.ox) * r.ix, t1x = (qplane(ox, fx, shift, sx) - r.ox) * r.ix;
-    const float t0y = (qplane(oy, ny, shift, sy) - r.oy) * r.iy, t1y = (qplane(oy, fy, shift, sy) - r.oy) * r.iy;
-    const float t0z = (qplane(oz, nz, shift, sz) - r.oz) * r.iz, t1z = (qplane(oz, fz, shift, sz) - r.oz) * r.iz;
+    const float t0x = __builtin_fmaf((float)((nx >> shift) & 0xffu), ax, nbx);
+    const float t1x = __builtin_fmaf((float)((fx >> shift) & 0xffu), ax, fbx);
+    const float t0y = __builtin_fmaf((float)((ny >> shift) & 0xffu), ay, nby);
+    const float t1y = __builtin_fmaf((float)((fy >> shift) & 0xffu), ay, fby);
+    const float t0z = __builtin_fmaf((float)((nz >> shift) & 0xffu), az, nbz);
+    const float t1z = __builtin_fmaf((float)((fz >> shift) & 0xffu), az, fbz);
     lo = fmaxf(fmaxf(t0x, t0y), t0z);
     const float hi = fminf(fminf(t1x, t1y), t1z);
     return (lo <= hi) & (lo <= R1) & (hi >= r.R0) & (child != MCRT_CHILD_EMPTY);
@@ -260,10 +289,16 @@ struct Tracer
                 const uint32_t nx = r.sx ? qa.y : qa.x, fx = r.sx ? qa.x : qa.y;
                 const uint32_t ny = r.sy ? qa.w : qa.z, fy = r.sy ? qa.z : qa.w;
                 const uint32_t nz = r.sz ? qb.y : qb.x, fz = r.sz ? qb.x : qb.y;
-                p0 = qslab_nf(r, ch.x, nx, fx, ny, fy, nz, fz, ox, oy, oz, sx, sy, sz, 0, R1, l0);
-                p1 = qslab_nf(r, ch.y, nx, fx, ny, fy, nz, fz, ox, oy, oz, sx, sy, sz, 8, R1, l1);
-                p2 = qslab_nf(r, ch.z, nx, fx, ny, fy, nz, fz, ox, oy, oz, sx, sy, sz, 16, R1, l2);
-                p3 = qslab_nf(r, ch.w, nx, fx, ny, fy, nz, fz, ox, oy, oz, sx, sy, sz, 24, R1, l3);
+                // the per-axis plane maps (node_margin), shared by the four children
+                const float bx = (ox - r.ox) * r.ix, by = (oy - r.oy) * r.iy, bz = (oz - r.oz) * r.iz;
+                const float ax = sx * r.ix, ay = sy * r.iy, az = sz * r.iz;
+                const float nbx = __builtin_fmaf(-r.w, fabsf(r.ix), bx), fbx = __builtin_fmaf(r.w, fabsf(r.ix), bx);
+                const float nby = __builtin_fmaf(-r.w, fabsf(r.iy), by), fby = __builtin_fmaf(r.w, fabsf(r.iy), by);
+                const float nbz = __builtin_fmaf(-r.w, fabsf(r.iz), bz), fbz = __builtin_fmaf(r.w, fabsf(r.iz), bz);
+                p0 = qslab_c(r, ch.x, nx, fx, ny, fy, nz, fz, ax, ay, az, nbx, fbx, nby, fby, nbz, fbz, 0, R1, l0);
+                p1 = qslab_c(r, ch.y, nx, fx, ny, fy, nz, fz, ax, ay, az, nbx, fbx, nby, fby, nbz, fbz, 8, R1, l1);
+                p2 = qslab_c(r, ch.z, nx, fx, ny, fy, nz, fz, ax, ay, az, nbx, fbx, nby, fby, nbz, fbz, 16, R1, l2);
+                p3 = qslab_c(r, ch.w, nx, fx, ny, fy, nz, fz, ax, ay, az, nbx, fbx, nby, fby, nbz, fbz, 24, R1, l3);
             }
             else
             {
@@ -408,7 +443,9 @@ MCRT_HD int trace(const DevScene& S, TraceStack& st, const RayPrep& r, float tma
                   float& t_out, float& u_out, float& v_out, TraceCount& cnt)
 {
     Tracer<ANY, COUNT> tr;
-    tr.start(r, tmax);
+    RayPrep rm = r;
+    node_margin(rm, S.node_span);
+    tr.start(rm, tmax);
     if (S.nodes != nullptr)
         while (!tr.step(S, st, cnt)) {}
     return tr.result(t_out, u_out, v_out);

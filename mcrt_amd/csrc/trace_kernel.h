@@ -45,7 +45,7 @@ struct TraceBatch
 };
 
 // With an 8-entry stack the kernel is compiled for at least seven waves per
-// SIMD (28 per CU, with LDS for all of them; the closest-hit kernel takes 57
+// SIMD (28 per CU, with LDS for all of them; the closest-hit kernel takes 59
 // VGPRs).  Held to eight waves' budget the compiler spilled SGPRs into VGPR
 // lanes, and the spill code in the refill block cost more than the eighth
 // wave gained.
@@ -86,6 +86,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
     root.nodes = S.nodes;
     root.refs = S.refs;
     root.det_eps = S.det_eps;
+    root.node_span = S.node_span;
     uint32_t i = 0;
     const uint32_t* order = ORDER && *B.skip == 0 ? B.order : nullptr;
     uint32_t claim = 0, claim_end = 0;  // wave-uniform: indices [claim, claim_end) not yet started
@@ -144,7 +145,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
                 i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : (ORDER && order != nullptr) ? order[k] : k;
                 if (BATCH) root = B.roots[B.rsid[i]];
                 const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
-                tr.start(prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w), b.w);
+                RayPrep rp = prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w);
+                node_margin(rp, root.node_span);
+                tr.start(rp, b.w);
                 live = true;
             }
             const uint32_t nidle = (uint32_t)__popcll(idle);

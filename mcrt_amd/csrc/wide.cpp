@@ -268,7 +268,7 @@ namespace
 {
 // One axis of a node: origin = lowest child plane; scale 2^k with the
 // smallest k in [-126, 127] at which every child's planes round outwards
-// within 8 bits (checked with the kernels' decode expression).
+// within 8 bits (checked with the decode expression, layout.h: qnode_plane).
 bool quantize_axis(const float* lo, const float* hi, const bool* valid, float& origin, uint32_t& ebyte,
                    uint32_t& qlo, uint32_t& qhi)
 {
@@ -329,6 +329,23 @@ bool quantize_axis(const float* lo, const float* hi, const bool* valid, float& o
     return false;
 }
 }  // namespace
+
+float qnode_span(const std::vector<QNode>& q)
+{
+    double g = 0.0;
+    for (const QNode& n : q)
+        for (int a = 0; a < 3 && (n.exps >> 24) != 0; a++)  // nodes with children
+        {
+            const double sc = std::ldexp(1.0, (int)((n.exps >> (8 * a)) & 0xffu) - 127);
+            const double v = std::fabs((double)n.origin[a]) + 255.0 * sc;
+            if (!(v <= (double)std::numeric_limits<float>::max())) return std::numeric_limits<float>::infinity();
+            g = std::max(g, v);
+        }
+    // round up to a float
+    float f = (float)g;
+    if ((double)f < g) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+    return f;
+}
 
 void quantize_nodes(WideBVH& out)
 {

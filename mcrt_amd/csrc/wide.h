@@ -20,4 +20,10 @@ void collapse_wide(const BTree& bt, const std::vector<Tri>& tris, WideBVH& out);
 
 // Fills out.qnodes from out.nodes (collapse_wide calls it).
 void quantize_nodes(WideBVH& out);
+
+// An upper bound, over every node and axis, of |origin| + 255 * scale (so
+// of the magnitude of every decoded plane and of 255 quantisation steps):
+// the scale of the rounding-error margin of the kernels' node tests
+// (trace_core.h: node_margin).  +inf if a node has non-finite planes.
+float qnode_span(const std::vector<QNode>& q);
 }  // namespace mcrt

@@ -98,6 +98,7 @@ def hostemu():
         "emu_build": (P, [P, LL, I]),
         "emu_digest": (U64, [P, P]),
         "emu_free": (None, [P]),
+        "emu_set_eps": (None, [P, F]),
         "emu_check_qnodes": (None, [P, P]),
         "emu_trace": (None, [P, P, LL, P, P, I, I, P, I]),
     })
@@ -281,3 +282,69 @@ def soup_slice(n_total=1 << 22, seed=3, lo=0.0, hi=0.25):
     keep = np.all((c >= lo) & (c < hi), axis=1)
     return np.ascontiguousarray(t[keep])
 
+
+
+def adversarial_rays(tris, n, seed=0):
+    """Rays chosen to stress the rounding margins of the kernels'
+    conservative node tests (trace_core.h: node_margin), as (n, 8) float32:
+    origins far outside the scene (10 to 10^6 scene extents away) aimed at
+    random triangles; origins placed exactly on vertex coordinates (the
+    planes of leaf boxes and, nearby, of quantised node boxes); directions
+    with components scaled down to 1e-8 .. 1e-38 (subnormal) or exactly 0;
+    very long and very short direction vectors (|d| from 1e-30 to 1e30,
+    whose inv_dir leave the margins' bounds); and random tmin / tmax."""
+    rng = np.random.default_rng(seed)
+    p = np.asarray(tris, np.float64).reshape(-1, 3, 3)
+    lo, hi = p.reshape(-1, 3).min(0), p.reshape(-1, 3).max(0)
+    ext = float(np.max(hi - lo)) or 1.0
+    k = rng.integers(0, p.shape[0], n)
+    bary = rng.dirichlet([1.0, 1.0, 1.0], n)
+    target = np.einsum("nk,nkj->nj", bary, p[k])
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = rng.uniform(lo - 0.2 * ext, hi + 0.2 * ext, (n, 3))
+    kind = rng.integers(0, 6, n)
+    far = kind == 0  # far away, aimed at a triangle
+    dist = ext * 10.0 ** rng.uniform(1, 6, n)
+    o[far] = target[far] - d[far] * dist[far, None]
+    on = kind == 1  # origin components on vertex coordinates
+    vk = p[rng.integers(0, p.shape[0], n), rng.integers(0, 3, n)]
+    mask = rng.random((n, 3)) < 0.7
+    o[on] = np.where(mask[on], vk[on], o[on])
+    d[on] = target[on] - o[on]
+    tiny = kind == 2  # one or two direction components nearly or exactly zero
+    scale = 10.0 ** -rng.choice([8, 12, 20, 30, 38, 45, 60], (n, 3))
+    pick = rng.random((n, 3)) < 0.5
+    d[tiny] = np.where(pick[tiny], d[tiny] * scale[tiny], d[tiny])
+    zero = rng.random(n) < 0.3
+    d[tiny & zero, rng.integers(0, 3)] = 0.0
+    mag = kind == 3  # unnormalised directions, 1e-30 .. 1e30 long
+    d[mag] *= 10.0 ** rng.uniform(-30, 30, (int(mag.sum()), 1))
+    aim = kind == 4  # aimed at a triangle from inside the box
+    d[aim] = target[aim] - o[aim]
+    r = np.empty((n, 8), np.float32)
+    with np.errstate(over="ignore", under="ignore", invalid="ignore"):
+        r[:, 0:3] = o
+        r[:, 4:7] = d
+    r[:, 3] = np.where(rng.random(n) < 0.2, rng.uniform(-1.0, 0.5, n) * ext, 1e-4 * min(1.0, ext)).astype(np.float32)
+    r[:, 7] = np.where(rng.random(n) < 0.2, rng.uniform(0.0, 3.0, n) * ext, np.inf).astype(np.float32)
+    return r
+
+
+def margin_scene(which):
+    """Scenes of the node-margin tests: the DebugMesh, the Cornell box with
+    an icosphere, and a soup slice translated far from the origin, scaled
+    down by 1e-4 and scaled up by 1e4."""
+    if which == "debug_mesh":
+        return debug_mesh()
+    if which == "cornell_ico4":
+        from mcrt_amd import scenes
+        return scenes.cornell_icosphere(4).tris
+    t = soup_slice(n_total=1 << 18, hi=0.5).astype(np.float64).reshape(-1, 3)
+    if which == "soup_far":
+        t = t + np.array([1000.0, -2000.0, 500.0])
+    elif which == "soup_tiny":
+        t = t * 1e-4
+    elif which == "soup_huge":
+        t = t * 1e4
+    return np.ascontiguousarray(t.reshape(-1, 9).astype(np.float32))

@@ -22,6 +22,7 @@ struct Emu
     WideBVH wb;
     uint64_t digest = 0;
     long long stats[8] = {0};
+    float det_eps = 1e-6f;  // Moller-Trumbore determinant cut of the traversal
 };
 }  // namespace
 
@@ -62,6 +63,8 @@ extern "C" unsigned long long emu_digest(void* h, long long* stats)
 }
 
 extern "C" void emu_free(void* h) { delete (Emu*)h; }
+
+extern "C" void emu_set_eps(void* h, float eps) { ((Emu*)h)->det_eps = eps; }
 
 // Quantised nodes against the exact ones: out[0] = planes checked, out[1] =
 // decoded planes that do not contain the exact child box, out[2] = child
@@ -122,6 +125,8 @@ extern "C" void emu_trace(void* h, const float* rays, long long n, int* hits, un
     S.nodes = e->wb.qnodes.empty() ? nullptr : e->wb.qnodes.data();
     S.refs = e->wb.refs.data();
     S.stack_bound = e->wb.stack_bound;
+    S.node_span = qnode_span(e->wb.qnodes);
+    S.det_eps = e->det_eps;
     // std::thread, not OpenMP: this TU is compiled by hipcc, whose OpenMP
     // runtime differs from the libgomp that the rest of the process uses.
     const int nt = std::max(1, threads);

@@ -197,3 +197,34 @@ def test_batch_sizes_around_work_claims(ctx, n):
     for waves in (1, 3):
         s.set_tuning(trace_waves_per_cu=waves)
         assert np.array_equal(mcrt_amd.split_hits(s.intersect(rays))[0], oprim)
+
+
+@pytest.mark.parametrize("which", ["debug_mesh", "cornell_ico4", "soup_far", "soup_tiny", "soup_huge"])
+def test_conservative_node_tests_adversarial(ctx, which):
+    """The traversal tests node boxes with a cheaper plane map and a per-ray
+    rounding margin (trace_core.h: node_margin).  Rays built to stress the
+    margin (far origins, origins on vertex coordinates, nearly zero or zero
+    direction components, directions 1e-30 to 1e30 long, random tmin /
+    tmax) over scenes near the origin, translated far from it, and scaled
+    down and up, give the oracle's hits bit for bit, closest and any hit."""
+    import mcrt_amd
+    tris = _libs.margin_scene(which)
+    rays = _libs.adversarial_rays(tris, 1 << 16, seed=12)
+    eps = 1e-20 if which == "soup_tiny" else 1e-6
+    print(f"[adversarial {which}] ntri={len(tris)} nrays={len(rays)} eps={eps}", flush=True)
+    s, st = _scene(ctx, tris)
+    ob = _libs.OracleBVH(tris)
+    assert st["tree_digest"] == ob.digest()[0]
+    s.set_intersection_epsilon(eps)
+    _libs.oracle().orc_set_intersection_epsilon(eps)
+    try:
+        oprim, otuv = ob.intersect(rays)
+    finally:
+        _libs.oracle().orc_set_intersection_epsilon(1e-6)
+    assert (oprim >= 0).sum() > len(rays) // 10
+    prim, tuv = mcrt_amd.split_hits(s.intersect(rays))
+    bad = np.nonzero(prim != oprim)[0]
+    assert bad.size == 0, f"{bad.size} primitive mismatches, first rays {bad[:5]}"
+    assert np.array_equal(tuv.view(np.uint32), otuv.view(np.uint32))
+    assert np.array_equal(s.occluded(rays), (oprim >= 0).astype(np.uint8))
+    s.close()

@@ -171,3 +171,36 @@ def test_c3_soup_full_size_builder_matches_oracle(emu):
     rays = np.ascontiguousarray(scenes.c3_rays("incoherent")[::stride])
     hits, _ = e.trace(rays, lds=12)
     assert np.array_equal(hits, np.asarray(G["sets"]["incoherent"]["sample_hits"], np.int32))
+
+
+@pytest.mark.parametrize("which", ["debug_mesh", "cornell_ico4", "soup_far", "soup_tiny", "soup_huge"])
+def test_conservative_node_tests_adversarial(emu, which):
+    """The kernels test node boxes with a cheaper plane map and a per-ray
+    rounding margin (trace_core.h: node_margin); a node test failing where
+    the reference's arithmetic on the decoded box passes would drop hits.
+    Rays built to stress the margin (far origins, origins on vertex
+    coordinates, nearly zero or zero direction components, directions
+    1e-30 to 1e30 long, random tmin / tmax), over scenes near the origin,
+    translated far from it, and scaled down and up, give the oracle's hits
+    bit for bit, closest and any hit."""
+    tris = _libs.margin_scene(which)
+    e = Emu(tris)
+    ob = _libs.OracleBVH(tris)
+    assert e.digest()[0] == ob.digest()[0]
+    rays = _libs.adversarial_rays(tris, 1 << 15, seed=11)
+    # the reference's determinant cut is absolute (1e-6): the scaled-down
+    # soup's triangles need a smaller one to be hit at all
+    eps = 1e-20 if which == "soup_tiny" else 1e-6
+    e.L.emu_set_eps(e.h, eps)
+    _libs.oracle().orc_set_intersection_epsilon(eps)
+    try:
+        p, tuv = ob.intersect(rays)
+    finally:
+        _libs.oracle().orc_set_intersection_epsilon(1e-6)
+    assert (p >= 0).sum() > len(rays) // 10
+    for lds in (16, 2):
+        hits, _ = e.trace(rays, lds=lds)
+        assert np.array_equal(hits[:, 0], p)
+        assert np.array_equal(hits[:, 1:].view(np.uint32), tuv.view(np.uint32))
+    occ, _ = e.trace(rays, any_hit=True)
+    assert np.array_equal(occ, (p >= 0).astype(np.uint8))
