@@ -516,7 +516,7 @@ def main_leg(args, scene, data):
         # what limits the kernel, measured: the fraction of the CUs' vector
         # issue slots it uses (the algorithmic-byte fraction above is the
         # contract's roofline; the traffic beyond L2 is well below HBM peak)
-        out["roofline"]["measured_limit"] = {"kind": "vector issue", "valu_issue_frac": prof["valu_issue_frac"],
+        out["roofline"]["measured_limit"] = {"kind": "instruction issue", "valu_issue_frac": prof["valu_issue_frac"],
                                              "valu_insts_per_ray": prof.get("valu_insts_per_ray"),
                                              "source": os.path.relpath(PROFILE, ROOT)}
     if not (args.no_cpu or args.profile):
