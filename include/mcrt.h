@@ -307,6 +307,8 @@ typedef struct mcrt_tuning {
                                       lanes once at least this many of its 64 lanes
                                       are idle, so refills come in fuller rounds    */
     int32_t pt_trace_refill_lanes; /* the same for the traversal launches of renders (16) */
+    int32_t trace_any_refill_lanes; /* the same for any-hit queries (mcrt_occluded), whose
+                                       rays end at their first hit (32)            */
 } mcrt_tuning;
 MCRT_API int mcrt_scene_set_tuning(mcrt_scene* scene, const mcrt_tuning* tuning);
 MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* scene, mcrt_tuning* tuning);
@@ -314,7 +316,8 @@ MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* scene, mcrt_tuning* tuning)
 /* Autotuner (no reference counterpart): times closest-hit over the n rays
  * for trace_waves_per_cu in {8,16,24,28,64,128} x trace_lds_depth in
  * {8,12,16} and keeps the fastest, then trace_refill_lanes in {1,8,16,32}
- * with the chosen pair, then trace_reorder_rays -1 against min(n, 2^20)
+ * with the chosen pair, then trace_any_refill_lanes in {16,32,48} (any-hit
+ * over the same rays), then trace_reorder_rays -1 against min(n, 2^20)
  * (the caller's ray order against origin-cell order), keeping the fastest
  * each time; with `render` != NULL also times one render per
  * pt_trace_waves_per_cu in {16,28,64} x pt_trace_lds_depth in {8,16}, then

@@ -51,7 +51,8 @@ class Tuning(ctypes.Structure):
     _fields_ = [("trace_waves_per_cu", ctypes.c_int32), ("trace_lds_depth", ctypes.c_int32),
                 ("pt_chunk_paths", ctypes.c_int64), ("pt_trace_waves_per_cu", ctypes.c_int32),
                 ("pt_trace_lds_depth", ctypes.c_int32), ("trace_reorder_rays", ctypes.c_int64),
-                ("trace_refill_lanes", ctypes.c_int32), ("pt_trace_refill_lanes", ctypes.c_int32)]
+                ("trace_refill_lanes", ctypes.c_int32), ("pt_trace_refill_lanes", ctypes.c_int32),
+                ("trace_any_refill_lanes", ctypes.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -330,11 +331,12 @@ class Scene:
         return dict(zip(("nodes", "refs", "tris", "rays"), (int(x) for x in c)))
 
     def set_tuning(self, trace_waves_per_cu=0, trace_lds_depth=0, pt_chunk_paths=0, pt_trace_waves_per_cu=0,
-                   pt_trace_lds_depth=0, trace_reorder_rays=0, trace_refill_lanes=0, pt_trace_refill_lanes=0):
+                   pt_trace_lds_depth=0, trace_reorder_rays=0, trace_refill_lanes=0, pt_trace_refill_lanes=0,
+                   trace_any_refill_lanes=0):
         """Launch tuning (0 keeps a field; trace_reorder_rays=-1: never
         reorder queries); results never depend on it."""
         t = Tuning(trace_waves_per_cu, trace_lds_depth, pt_chunk_paths, pt_trace_waves_per_cu, pt_trace_lds_depth,
-                   trace_reorder_rays, trace_refill_lanes, pt_trace_refill_lanes)
+                   trace_reorder_rays, trace_refill_lanes, pt_trace_refill_lanes, trace_any_refill_lanes)
         _check(lib().mcrt_scene_set_tuning(self.h, ctypes.byref(t)))
 
     def tuning(self):

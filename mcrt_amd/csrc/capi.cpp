@@ -506,6 +506,8 @@ static int check_tuning(const mcrt_tuning* t)
         return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_refill_lanes in [1, 64] (0 = keep)");
     if (t->pt_trace_refill_lanes < 0 || t->pt_trace_refill_lanes > 64)
         return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: pt_trace_refill_lanes in [1, 64] (0 = keep)");
+    if (t->trace_any_refill_lanes < 0 || t->trace_any_refill_lanes > 64)
+        return fail(MCRT_ERR_INVALID_ARG, "mcrt_tuning: trace_any_refill_lanes in [1, 64] (0 = keep)");
     return MCRT_OK;
 }
 
@@ -522,6 +524,7 @@ MCRT_API int mcrt_scene_set_tuning(mcrt_scene* s, const mcrt_tuning* t)
     if (t->trace_reorder_rays) s->trace_reorder_rays = t->trace_reorder_rays;
     if (t->trace_refill_lanes) s->trace_refill_lanes = t->trace_refill_lanes;
     if (t->pt_trace_refill_lanes) s->pt_trace_refill_lanes = t->pt_trace_refill_lanes;
+    if (t->trace_any_refill_lanes) s->trace_any_refill_lanes = t->trace_any_refill_lanes;
     if (s->finalized) apply_trace_tuning(s);
     return MCRT_OK;
 }
@@ -537,6 +540,7 @@ MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* s, mcrt_tuning* t)
     t->trace_reorder_rays = s->trace_reorder_rays;
     t->trace_refill_lanes = s->trace_refill_lanes;
     t->pt_trace_refill_lanes = s->pt_trace_refill_lanes;
+    t->trace_any_refill_lanes = s->trace_any_refill_lanes;
     return MCRT_OK;
 }
 
@@ -562,6 +566,7 @@ MCRT_API int mcrt_autotune(mcrt_scene* s, const float* rays, size_t n, const mcr
         s->trace_reorder_rays = before.trace_reorder_rays;
         s->trace_refill_lanes = before.trace_refill_lanes;
         s->pt_trace_refill_lanes = before.pt_trace_refill_lanes;
+        s->trace_any_refill_lanes = before.trace_any_refill_lanes;
         if (s->finalized) apply_trace_tuning(s);
         fail(rc, "%s", err.c_str());
     }
@@ -656,6 +661,36 @@ static int autotune(mcrt_scene* s, const float* rays, size_t n, const mcrt_rende
             }
         }
         s->trace_refill_lanes = br;
+        // then the any-hit queries' threshold, on the same rays
+        {
+            DevBuf d_occ;
+            if ((rc = d_occ.ensure(n))) return rc;
+            static const int kAnyRefill[] = {16, 32, 48};
+            int ba = s->trace_any_refill_lanes;
+            double any_rate = 0.0;
+            for (int k : kAnyRefill)
+            {
+                s->trace_any_refill_lanes = k;
+                apply_trace_tuning(s);
+                DevScene view;
+                if ((rc = launch_view(s, res, view, (int64_t)n))) return rc;
+                uint8_t* o = d_occ.as<uint8_t>();
+                HIPCHK(launch_trace(view, r, (int64_t)n, nullptr, o, TRACE_ANY, nullptr, st));
+                HIPCHK(hipEventRecord(e0, st));
+                for (int j = 0; j < 3; j++) HIPCHK(launch_trace(view, r, (int64_t)n, nullptr, o, TRACE_ANY, nullptr, st));
+                HIPCHK(hipEventRecord(e1, st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                const double rate = 3.0 * (double)n / (ms * 1e-3) / 1e6;
+                if (rate > any_rate)
+                {
+                    any_rate = rate;
+                    ba = k;
+                }
+            }
+            s->trace_any_refill_lanes = ba;
+        }
         // then the ray order: the caller's, or by origin cell from min(n, 2^20) rays
         const int64_t order[2] = {-1, (int64_t)n < (int64_t(1) << 20) ? (int64_t)n : int64_t(1) << 20};
         double order_rate[2] = {0.0, 0.0};

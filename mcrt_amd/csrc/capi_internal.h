@@ -161,6 +161,10 @@ struct mcrt_scene
     // queries; results do not depend on it
     int32_t trace_refill_lanes = 16;
     int32_t pt_trace_refill_lanes = 16;
+    // any-hit queries' threshold: their rays end at the first hit, so lanes
+    // free up faster; 32 measured fastest (the 1.3M-triangle mesh's 16M rays
+    // 0.724 against 0.749 ms at 16, the soup's 6.43 against 7.09 ms)
+    int32_t trace_any_refill_lanes = 32;
 };
 
 namespace mcrt

@@ -44,6 +44,7 @@ struct DevScene
     const int32_t* hot_tri = nullptr;  // triangle of each gradient hot slot
     int32_t nhot = 0;
     int32_t refill_lanes = 1;          // a traversal wave claims new rays once this many of its lanes are idle
+    int32_t refill_lanes_any = 1;      // the same for any-hit queries
     float det_eps = 1e-6f;             // Moller-Trumbore |det| cut (TriangleObject::s_intersection_epsilon)
     float node_span = 0.0f;            // qnode_span of the nodes (wide.h): scale of the node-test margin
     int64_t ntri = 0;

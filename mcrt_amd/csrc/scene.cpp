@@ -617,6 +617,7 @@ void apply_trace_tuning(mcrt_scene* s)
     trace_geometry(s->dev, s->ctx->num_cus, s->trace_waves_per_cu, s->trace_lds_depth);
     s->dev.reorder_rays = s->trace_reorder_rays > 0 ? s->trace_reorder_rays : 0;
     s->dev.refill_lanes = s->trace_refill_lanes;
+    s->dev.refill_lanes_any = s->trace_any_refill_lanes;
     s->dev.det_eps = s->det_eps;
 }
 

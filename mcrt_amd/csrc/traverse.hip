@@ -217,6 +217,10 @@ hipError_t launch_trace(const DevScene& s, const float* rays, int64_t n, void* h
                         uint64_t* counters, hipStream_t stream)
 {
     const bool any = mode == TRACE_ANY;
+    // any-hit queries have their own compaction threshold (their rays end at
+    // the first hit, so lanes free up faster)
+    DevScene sa = s;
+    if (any) sa.refill_lanes = s.refill_lanes_any;
     const int64_t kMaxLaunch = int64_t(1) << 30;  // the kernel indexes rays with 32 bits
     for (int64_t off = 0; off < n; off += kMaxLaunch)
     {
@@ -224,10 +228,10 @@ hipError_t launch_trace(const DevScene& s, const float* rays, int64_t n, void* h
         const float* r = rays + 8 * off;
         void* h = any ? nullptr : static_cast<void*>(static_cast<int4*>(hits) + off);
         uint8_t* o = any ? occ + off : nullptr;
-        hipError_t e = counters ? (any ? launch<true, true>(s, r, m, h, o, counters, stream)
-                                       : launch<false, true>(s, r, m, h, o, counters, stream))
-                                : (any ? launch<true, false>(s, r, m, h, o, counters, stream)
-                                       : launch<false, false>(s, r, m, h, o, counters, stream));
+        hipError_t e = counters ? (any ? launch<true, true>(sa, r, m, h, o, counters, stream)
+                                       : launch<false, true>(sa, r, m, h, o, counters, stream))
+                                : (any ? launch<true, false>(sa, r, m, h, o, counters, stream)
+                                       : launch<false, false>(sa, r, m, h, o, counters, stream));
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

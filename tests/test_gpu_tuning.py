@@ -26,12 +26,16 @@ def _debug_scene(ctx):
 def test_traversal_tuning_bit_exact(ctx, depth, waves):
     import mcrt_amd
     refill = {1: 64, 16: 1, 40: 7, 300: 16}[waves]  # compaction thresholds, one per row
-    print(f"[tuning] depth={depth} waves={waves} refill={refill}", flush=True)
+    any_refill = {1: 1, 16: 64, 40: 32, 300: 5}[waves]
+    print(f"[tuning] depth={depth} waves={waves} refill={refill} any_refill={any_refill}", flush=True)
     tris, s, st = _debug_scene(ctx)
     assert st["stack_bound"] > 16, "fixture spills at LDS depths 8, 12 and 16"
-    s.set_tuning(trace_waves_per_cu=waves, trace_lds_depth=depth, trace_refill_lanes=refill)
+    assert s.tuning()["trace_any_refill_lanes"] == 32
+    s.set_tuning(trace_waves_per_cu=waves, trace_lds_depth=depth, trace_refill_lanes=refill,
+                 trace_any_refill_lanes=any_refill)
     t = s.tuning()
     assert (t["trace_lds_depth"], t["trace_waves_per_cu"], t["trace_refill_lanes"]) == (depth, waves, refill)
+    assert t["trace_any_refill_lanes"] == any_refill
     rays = _libs.random_rays(tris, 1 << 15, seed=11)
     prim, tuv = mcrt_amd.split_hits(s.intersect(rays))
     op, otuv = _libs.OracleBVH(tris).intersect(rays)
@@ -126,6 +130,7 @@ def test_autotune_applies_valid_tuning(ctx):
     assert t["pt_chunk_paths"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
     assert t["pt_trace_waves_per_cu"] in (16, 28, 64) and t["pt_trace_lds_depth"] in (8, 16)
     assert t["trace_refill_lanes"] in (1, 8, 16, 32) and t["pt_trace_refill_lanes"] in (1, 8, 16, 32)
+    assert t["trace_any_refill_lanes"] in (16, 32, 48)
     assert t["trace_reorder_rays"] in (-1, rays.shape[0])
     assert rate > 0
     assert np.array_equal(s.intersect(rays), before)
@@ -137,7 +142,8 @@ def test_tuning_errors(ctx):
     _, s, _ = _debug_scene(ctx)
     for kw in (dict(trace_lds_depth=10), dict(trace_waves_per_cu=1025), dict(trace_waves_per_cu=-1),
                dict(pt_chunk_paths=(1 << 28) + 1), dict(pt_trace_lds_depth=4), dict(pt_trace_waves_per_cu=2000),
-               dict(trace_reorder_rays=-2), dict(trace_refill_lanes=65), dict(pt_trace_refill_lanes=-1)):
+               dict(trace_reorder_rays=-2), dict(trace_refill_lanes=65), dict(pt_trace_refill_lanes=-1),
+               dict(trace_any_refill_lanes=65), dict(trace_any_refill_lanes=-3)):
         with pytest.raises(mcrt_amd.McrtError) as e:
             s.set_tuning(**kw)
         assert e.value.code == 1
