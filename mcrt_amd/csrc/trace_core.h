@@ -12,12 +12,14 @@
 // through `lo <= tmax`, so re-checking `lo` when the entry is popped is the
 // reference's entry-time test exactly.  Any-hit stops at the first accepted
 // hit, which is the first hit of the same traversal, so its boolean equals
-// SceneBVH::occluded's (scenebvh.cpp:379-396).
+// SceneBVH::occluded's (scenebvh.cpp:379-396).  Node tests only have to be
+// conservative (node_margin below); the re-test of a popped entry uses the
+// conservative `lo`, which passes wherever the reference's passes.
 //
-// Arithmetic is the reference's: slab t = (bound - o) * inv_dir with
+// Leaf arithmetic is the reference's: slab t = (bound - o) * inv_dir with
 // inv_dir = (d == 0 ? 0 : 1/d) (ray.cpp:77-84), MT with the absolute
-// 1e-6 determinant cut (triangle.cpp:13-51); all fp32 with no contraction
-// and correctly rounded division.
+// determinant cut (1e-6 unless set; triangle.cpp:13-56); all fp32 with no
+// contraction and correctly rounded division.
 #pragma once
 #include <hip/hip_runtime.h>
 
