@@ -10,11 +10,12 @@
 // node tests its children when it is visited and a child's entry distance
 // `lo` travels on the stack: the reference's box test depends on tmax only
 // through `lo <= tmax`, so re-checking `lo` when the entry is popped is the
-// reference's entry-time test exactly.  Any-hit stops at the first accepted
+// reference's entry-time test (conservative, as every node test is: see
+// below).  Any-hit stops at the first accepted
 // hit, which is the first hit of the same traversal, so its boolean equals
 // SceneBVH::occluded's (scenebvh.cpp:379-396).  Node tests only have to be
-// conservative (node_margin below); the re-test of a popped entry uses the
-// conservative `lo`, which passes wherever the reference's passes.
+// conservative (node_margin below): the `lo` carried on the stack is at most
+// the reference's, so its re-test passes wherever the reference's passes.
 //
 // Leaf arithmetic is the reference's: slab t = (bound - o) * inv_dir with
 // inv_dir = (d == 0 ? 0 : 1/d) (ray.cpp:77-84), MT with the absolute
