@@ -228,3 +228,27 @@ def test_conservative_node_tests_adversarial(ctx, which):
     assert np.array_equal(tuv.view(np.uint32), otuv.view(np.uint32))
     assert np.array_equal(s.occluded(rays), (oprim >= 0).astype(np.uint8))
     s.close()
+
+
+def test_ties_and_degenerate_triangles(ctx):
+    """Exact ties in t (duplicated triangles, a coplanar grid hit head-on,
+    rays through shared edges) and zero-area triangles: the GPU keeps the
+    reference's first-visited hit among equal t, so ids are bit-exact."""
+    rng = np.random.default_rng(8)
+    grid = []
+    for i in range(20):
+        for j in range(20):
+            x, y = i * 0.05, j * 0.05
+            grid += [[x, y, 0, x + 0.05, y, 0, x, y + 0.05, 0], [x + 0.05, y, 0, x + 0.05, y + 0.05, 0, x, y + 0.05, 0]]
+    grid = np.array(grid, np.float32)
+    dup = np.concatenate([grid, grid[:50], grid[100:140]])
+    degen = np.concatenate([dup, np.repeat(rng.uniform(0, 1, (30, 3)), 3, axis=0).reshape(30, 9)]).astype(np.float32)
+    rays = _libs.random_rays(degen, 1 << 15, seed=3)
+    rays[::2, 4:7] = [0.0, 0.0, -1.0]  # half the rays straight down onto the plane
+    rays[::2, 2] = 0.5
+    rays[1::4, 0] = np.round(rays[1::4, 0] / 0.05) * 0.05  # origins on grid lines: shared edges
+    rays[1::4, 4:7] = [0.0, 0.0, -1.0]
+    rays[1::4, 2] = 0.5
+    s, _, oprim = _check_closest(ctx, degen, rays, "ties")
+    assert (oprim >= 0).sum() > len(rays) // 4
+    s.close()
