@@ -99,6 +99,7 @@ def hostemu():
         "emu_digest": (U64, [P, P]),
         "emu_free": (None, [P]),
         "emu_set_eps": (None, [P, F]),
+        "emu_margin_check": (None, [LL, U64, F, P, P]),
         "emu_check_qnodes": (None, [P, P]),
         "emu_trace": (None, [P, P, LL, P, P, I, I, P, I]),
     })

@@ -181,3 +181,78 @@ extern "C" void emu_trace(void* h, const float* rays, long long n, int* hits, un
     for (int t = 0; t < nt; t++) { cn += cnts[3 * t]; cr += cnts[3 * t + 1]; ct += cnts[3 * t + 2]; }
     if (counters) { counters[0] = cn; counters[1] = cr; counters[2] = ct; }
 }
+
+// The node-test margin (trace_core.h: node_margin) against the reference's
+// slab arithmetic on single planes: n random cases of node origin, scale
+// 2^k, plane code, ray origin and direction (magnitudes from tiny to huge,
+// zero direction components), each a node of span |origin| + 255 * 2^k
+// (the tightest span a scene can give it).  out[0] = cases checked (rays
+// the margin's bounds admit), out[1] = cases where the lower bound exceeds
+// or the upper bound falls below RN(RN(plane - o) * inv), out[2] = cases
+// sent to the exact path; worst = the largest |t_ref - (q A + B)| / (w |inv|)
+// seen (the margin's share the rounding used).  `wscale` multiplies the
+// margin (1 for the product's; below 1 to show the check catches a margin
+// that is too small).
+extern "C" void emu_margin_check(long long n, unsigned long long seed, float wscale, long long* out, double* worst)
+{
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto next = [&st]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return st;
+    };
+    auto unif = [&]() { return (double)(next() >> 11) * 0x1p-53; };
+    long long checked = 0, bad = 0, exact = 0;
+    double wmax = 0.0;
+    for (long long c = 0; c < n; c++)
+    {
+        const int k = (int)(unif() * 90.0) - 60;  // scale 2^k, k in [-60, 29]
+        const float s = std::ldexp(1.0f, k);
+        const float O = (float)((unif() * 2.0 - 1.0) * std::ldexp(1.0, k + (int)(unif() * 12.0)));
+        const uint32_t q = (uint32_t)(next() & 0xffu);
+        const float span = std::nextafter((float)(std::fabs((double)O) + 255.0 * (double)s), INFINITY);
+        float o[3], d[3];
+        for (int a = 0; a < 3; a++)
+        {
+            const double mag = std::fabs((double)O) + 255.0 * (double)s;
+            const int kind = (int)(next() % 4);
+            if (kind == 0)
+                o[a] = (float)((unif() * 2.0 - 1.0) * mag * std::pow(10.0, unif() * 6.0 - 3.0));
+            else if (kind == 1)
+                o[a] = std::fma((float)(next() & 0xffu), s, O);  // on a plane of the node
+            else
+                o[a] = (float)((double)O + (unif() * 2.0 - 1.0) * 300.0 * (double)s);
+            const int dk = (int)(next() % 8);
+            const double dm = dk == 0 ? 0.0 : std::pow(10.0, unif() * 60.0 - 30.0);
+            d[a] = (float)((next() & 1) ? dm : -dm);
+        }
+        RayPrep r = prep_ray(o[0], o[1], o[2], d[0], d[1], d[2], 1e-4f);
+        node_margin(r, span);
+        r.w *= wscale;
+        if (r.exact)
+        {
+            exact++;
+            continue;
+        }
+        const float oo = r.ox, inv = r.ix;  // the x axis carries the case
+        const float P = __builtin_fmaf((float)q, s, O);
+        const float tref = (P - oo) * inv;
+        const float A = s * inv;
+        const float B = (O - oo) * inv;
+        const float tn = __builtin_fmaf((float)q, A, __builtin_fmaf(-r.w, std::fabs(inv), B));
+        const float tf = __builtin_fmaf((float)q, A, __builtin_fmaf(r.w, std::fabs(inv), B));
+        checked++;
+        if (!(tn <= tref && tf >= tref)) bad++;
+        const double m = (double)r.w * std::fabs((double)inv);
+        if (m > 0.0)
+        {
+            const double e = std::fabs((double)tref - ((double)q * (double)A + (double)B)) / m;
+            if (e > wmax) wmax = e;
+        }
+    }
+    out[0] = checked;
+    out[1] = bad;
+    out[2] = exact;
+    *worst = wmax;
+}

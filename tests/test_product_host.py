@@ -2,6 +2,7 @@
 4-wide collapse (wide.cpp) and the traversal core the kernels execute
 (trace_core.h) -- run on the host through tests/hostemu, against the golden
 fixtures of the compiled reference and against the oracle.  CPU only."""
+import ctypes
 import os
 
 import numpy as np
@@ -204,3 +205,23 @@ def test_conservative_node_tests_adversarial(emu, which):
         assert np.array_equal(hits[:, 1:].view(np.uint32), tuv.view(np.uint32))
     occ, _ = e.trace(rays, any_hit=True)
     assert np.array_equal(occ, (p >= 0).astype(np.uint8))
+
+
+def test_node_margin_bounds_reference_planes(emu):
+    """The node-test margin on single planes (trace_core.h: node_margin),
+    case by case: 4M random combinations of node origin, scale 2^k
+    (k in [-60, 29]), plane code, ray origin (on, near and far from the
+    node's planes) and direction (1e-30 to 1e30 long, or zero) give a
+    lower bound at most, and an upper bound at least, the reference's
+    RN(RN(plane - o) * inv) wherever node_margin admits the ray.  The same
+    check with the margin cut to an eighth fails, so it can tell."""
+    out = (ctypes.c_longlong * 3)()
+    worst = ctypes.c_double()
+    emu.emu_margin_check(4_000_000, 7, 1.0, out, ctypes.byref(worst))
+    checked, bad, exact = list(out)
+    assert bad == 0
+    assert checked > 2_500_000 and exact > 0
+    # the rounding uses part of the margin, not all of it
+    assert 0.05 < worst.value < 0.75
+    emu.emu_margin_check(400_000, 7, 0.125, out, ctypes.byref(worst))
+    assert out[1] > 0
