@@ -126,7 +126,7 @@ struct mcrt_scene
     mcrt_build_stats stats{};
     // device residency
     mcrt::DevScene dev;
-    mcrt::DevBuf d_nodes, d_refs, d_shade, d_verts, d_mats, d_lights, d_hot;
+    mcrt::DevBuf d_bvh, d_shade, d_verts, d_mats, d_lights, d_hot;
     // Moller-Trumbore determinant cut (TriangleObject::setIntersectionEpsilon)
     float det_eps = 1e-6f;
     // environment map (dev.env_*); env_gen counts replacements

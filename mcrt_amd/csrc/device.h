@@ -33,8 +33,7 @@ struct DevCamera
 
 struct DevScene
 {
-    const QNode* nodes = nullptr;
-    const WRef* refs = nullptr;
+    const QNode* nodes = nullptr;      // the BVH's 64 B records: QNodes, then WRefs (layout.h)
     const TriShade* shade = nullptr;   // per triangle
     const float* verts = nullptr;      // per triangle a, b, c (9 floats)
     const DevMaterial* mats = nullptr;
@@ -81,8 +80,7 @@ struct DevScene
 // The part of a scene a traversal reads (batched launches look it up per ray).
 struct SceneRoot
 {
-    const QNode* nodes = nullptr;
-    const WRef* refs = nullptr;
+    const QNode* nodes = nullptr;  // the scene's BVH records (DevScene::nodes)
     float det_eps = 1e-6f;  // the scene's Moller-Trumbore determinant cut
     float node_span = 0.0f; // qnode_span of the scene's nodes
 };

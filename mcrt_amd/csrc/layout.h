@@ -18,6 +18,10 @@ namespace mcrt
 {
 // Child link: EMPTY, an interior wide node (index), or a leaf: bit 31 set,
 // bits 29-30 = count - 1 (1..4 references), bits 0-28 = first WRef.
+// On the device the QNodes and WRefs are one array of 64 B records, the
+// nodes first (device_records, wide.h), and a leaf link's bits 0-28 index
+// that array: the node count plus the first WRef's index.  A traversal step
+// then finds its record at one base whichever kind it visits.
 #define MCRT_CHILD_EMPTY 0xffffffffu
 #define MCRT_CHILD_LEAF 0x80000000u
 #define MCRT_LEAF_REF_MASK 0x1fffffffu

@@ -1017,7 +1017,6 @@ int batch_init(mcrt_batch* b)
         tab[i] = scene_pt(s, s->dev, &b->params[i]);
         tab[i].film = b->films[i];
         roots[i].nodes = s->dev.nodes;
-        roots[i].refs = s->dev.refs;
         roots[i].det_eps = s->dev.det_eps;
         roots[i].node_span = s->dev.node_span;
         pmax = std::max<int64_t>(pmax, b->params[i].spp);
@@ -1071,7 +1070,6 @@ int batch_init(mcrt_batch* b)
     DevScene& Dt = b->trace_view;
     Dt = s0->dev;
     Dt.nodes = nullptr;
-    Dt.refs = nullptr;
     Dt.stack_bound = max_sb;
     trace_geometry(Dt, b->ctx->num_cus, s0->pt_trace_waves_per_cu, s0->pt_trace_lds_depth);
     Dt.refill_lanes = s0->pt_trace_refill_lanes;

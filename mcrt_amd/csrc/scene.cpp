@@ -419,8 +419,20 @@ int finalize_scene(mcrt_scene* s, int threads, const char* cache)
     }
 
     int rc;
-    if ((rc = s->d_nodes.upload(wb.qnodes.data(), wb.qnodes.size() * sizeof(QNode)))) return rc;
-    if ((rc = s->d_refs.upload(wb.refs.data(), wb.refs.size() * sizeof(WRef)))) return rc;
+    {
+        // one array of 64 B records, nodes then references (layout.h)
+        std::vector<QNode> dn;
+        if (!device_records(wb, dn))
+            return fail(MCRT_ERR_OUT_OF_MEMORY, "BVH of %zu nodes and %zu references exceeds 2^29 records",
+                        wb.qnodes.size(), wb.refs.size());
+        const size_t nb = dn.size() * sizeof(QNode), rb = wb.refs.size() * sizeof(WRef);
+        if ((rc = s->d_bvh.ensure(nb + rb))) return rc;
+        hipError_t e = hipSuccess;
+        if (nb) e = hipMemcpy(s->d_bvh.p, dn.data(), nb, hipMemcpyHostToDevice);
+        if (e == hipSuccess && rb)
+            e = hipMemcpy(static_cast<char*>(s->d_bvh.p) + nb, wb.refs.data(), rb, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(MCRT_ERR_DEVICE, "hipMemcpy H2D: %s", hipGetErrorString(e));
+    }
     if ((rc = s->d_shade.upload(shade.data(), shade.size() * sizeof(TriShade)))) return rc;
     if ((rc = s->d_verts.upload(verts.data(), verts.size() * sizeof(float)))) return rc;
     if ((rc = s->d_lights.upload(lights.data(), lights.size() * sizeof(DevLight)))) return rc;
@@ -428,8 +440,7 @@ int finalize_scene(mcrt_scene* s, int threads, const char* cache)
     if ((rc = upload_materials(s))) return rc;
 
     DevScene& d = s->dev;
-    d.nodes = wb.qnodes.empty() ? nullptr : s->d_nodes.as<QNode>();
-    d.refs = s->d_refs.as<WRef>();
+    d.nodes = wb.qnodes.empty() ? nullptr : s->d_bvh.as<QNode>();
     d.shade = s->d_shade.as<TriShade>();
     d.verts = s->d_verts.as<float>();
     d.lights = s->d_lights.as<DevLight>();
@@ -472,14 +483,14 @@ int finalize_scene(mcrt_scene* s, int threads, const char* cache)
     // mesh: 2.67 instead of 2.08 ms).
     if (s->trace_reorder_rays == 0)
     {
-        const size_t bvh = s->d_nodes.bytes + s->d_refs.bytes;
+        const size_t bvh = s->d_bvh.bytes;
         s->trace_reorder_rays = bvh > (size_t(192) << 20) ? int64_t(1) << 20 : -1;
     }
     apply_trace_tuning(s);
     auto t2 = clk::now();
     st.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     st.flatten_seconds = std::chrono::duration<double>(t2 - t1).count();
-    st.device_bytes = (int64_t)(s->d_nodes.bytes + s->d_refs.bytes + s->d_shade.bytes + s->d_verts.bytes +
+    st.device_bytes = (int64_t)(s->d_bvh.bytes + s->d_shade.bytes + s->d_verts.bytes +
                                 s->d_lights.bytes + s->d_mats.bytes + s->d_hot.bytes);
     s->stats = st;
     return MCRT_OK;

@@ -46,7 +46,8 @@ struct RayPrep
                  // margins would leave their bounds): use the reference's
                  // sequential test with its NaN behaviour and descend every
                  // child of a node
-    uint32_t oct;
+    bool oct_hi;         // dz > 0: the octant's rank byte is in the node's upper rank word
+    uint32_t oct_shift;  // 8 * (octant & 3): its position in that word
 };
 
 MCRT_HD float inv_or_zero(float d) { return d == 0.0f ? 0.0f : 1.0f / d; }
@@ -62,7 +63,9 @@ MCRT_HD RayPrep prep_ray(float ox, float oy, float oz, float dx, float dy, float
     r.R0 = tmin;
     r.w = 0.0f;
     r.exact = __builtin_isinf(r.ix) || __builtin_isinf(r.iy) || __builtin_isinf(r.iz);
-    r.oct = (dx > 0.0f ? 1u : 0u) | (dy > 0.0f ? 2u : 0u) | (dz > 0.0f ? 4u : 0u);
+    // octant (dx > 0) | (dy > 0) << 1 | (dz > 0) << 2 of the nodes' rank tables
+    r.oct_hi = dz > 0.0f;
+    r.oct_shift = (dx > 0.0f ? 8u : 0u) | (dy > 0.0f ? 16u : 0u);
     return r;
 }
 
@@ -186,14 +189,16 @@ struct TraceStack
     uint32_t spill_lane;
     uint32_t spill_stride;  // lanes the spill area covers
 
-    // Unconditional LDS write to row k <= depth (row depth: discarded).
-    MCRT_HD void put(int k, uint32_t n, float l)
+    // Byte offset of this lane's entry in row k (of both arrays).
+    MCRT_HD uint32_t off(int k) const { return (uint32_t)k * (uint32_t)(stride * 4) + (uint32_t)(lane * 4); }
+    // Unconditional LDS write at off(k), k <= depth (row depth: discarded).
+    MCRT_HD void put(uint32_t o, uint32_t n, float l)
     {
-        node[k * stride + lane] = n;
-        lo[k * stride + lane] = l;
+        *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(node) + o) = n;
+        *reinterpret_cast<float*>(reinterpret_cast<char*>(lo) + o) = l;
     }
     // The same without the entry distance (any-hit never re-tests it).
-    MCRT_HD void put_node(int k, uint32_t n) { node[k * stride + lane] = n; }
+    MCRT_HD void put_node(uint32_t o, uint32_t n) { *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(node) + o) = n; }
     MCRT_HD void push(int sp, uint32_t n, float l)
     {
         if (sp < depth)
@@ -250,13 +255,13 @@ struct Tracer
         sp = 0;
     }
 
-    // The 64 B record `cur` names: a QNode, or a leaf reference.  Sc is any
-    // type with the scene's `nodes`, `refs` and `det_eps` (DevScene, SceneRoot).
+    // The 64 B record `cur` names: a QNode, or a leaf reference, both in the
+    // scene's one record array (layout.h).  Sc is any type with the scene's
+    // `nodes` and `det_eps` (DevScene, SceneRoot).
     template <class Sc>
     MCRT_HD const uint4* record(const Sc& S) const
     {
-        return (cur & MCRT_CHILD_LEAF) ? reinterpret_cast<const uint4*>(S.refs + (cur & MCRT_LEAF_REF_MASK))
-                                       : reinterpret_cast<const uint4*>(S.nodes + cur);
+        return reinterpret_cast<const uint4*>(S.nodes) + 4 * (size_t)(cur & MCRT_LEAF_REF_MASK);
     }
 
     // Visits `cur`; returns true once the traversal has ended.  Node and
@@ -319,7 +324,7 @@ struct Tracer
             // lands above as many entries as passing children follow it.
             // (Any-hit's boolean does not depend on the order, but this
             // near-first order finds occluders sooner than slot order.)
-            const uint32_t rk = ((r.oct & 4u) ? qb.w : qb.z) >> ((r.oct & 3u) * 8u);
+            const uint32_t rk = (r.oct_hi ? qb.w : qb.z) >> r.oct_shift;
             const uint32_t r0 = rk & 3u, r1 = (rk >> 2) & 3u, r2 = (rk >> 4) & 3u, r3 = (rk >> 6) & 3u;
             const uint32_t m = (p0 ? 1u << r0 : 0u) | (p1 ? 1u << r1 : 0u) | (p2 ? 1u << r2 : 0u) |
                                (p3 ? 1u << r3 : 0u);
@@ -327,28 +332,31 @@ struct Tracer
             {
                 const uint32_t first = (uint32_t)__builtin_ctz(m);
                 const int c = __builtin_popcount(m);
-                cur = (p0 && r0 == first) ? ch.x : (p1 && r1 == first) ? ch.y : (p2 && r2 == first) ? ch.z : ch.w;
                 const bool q0 = p0 && r0 != first, q1 = p1 && r1 != first;
                 const bool q2 = p2 && r2 != first, q3 = p3 && r3 != first;
-                const int i0 = sp + __builtin_popcount(m >> (r0 + 1u)), i1 = sp + __builtin_popcount(m >> (r1 + 1u));
-                const int i2 = sp + __builtin_popcount(m >> (r2 + 1u)), i3 = sp + __builtin_popcount(m >> (r3 + 1u));
+                // the nearest passing child is the passing one not pushed
+                cur = (p0 && !q0) ? ch.x : (p1 && !q1) ? ch.y : (p2 && !q2) ? ch.z : ch.w;
+                const uint32_t m1 = m >> 1;
+                const int i0 = sp + __builtin_popcount(m1 >> r0), i1 = sp + __builtin_popcount(m1 >> r1);
+                const int i2 = sp + __builtin_popcount(m1 >> r2), i3 = sp + __builtin_popcount(m1 >> r3);
                 if (sp + c - 1 <= st.depth)
                 {
                     // every push lands in LDS: four unconditional writes, those
                     // of slots not pushed into the discard row
+                    const uint32_t dis = st.off(st.depth);
                     if (ANY)
                     {
-                        st.put_node(q0 ? i0 : st.depth, ch.x);
-                        st.put_node(q1 ? i1 : st.depth, ch.y);
-                        st.put_node(q2 ? i2 : st.depth, ch.z);
-                        st.put_node(q3 ? i3 : st.depth, ch.w);
+                        st.put_node(q0 ? st.off(i0) : dis, ch.x);
+                        st.put_node(q1 ? st.off(i1) : dis, ch.y);
+                        st.put_node(q2 ? st.off(i2) : dis, ch.z);
+                        st.put_node(q3 ? st.off(i3) : dis, ch.w);
                     }
                     else
                     {
-                        st.put(q0 ? i0 : st.depth, ch.x, l0);
-                        st.put(q1 ? i1 : st.depth, ch.y, l1);
-                        st.put(q2 ? i2 : st.depth, ch.z, l2);
-                        st.put(q3 ? i3 : st.depth, ch.w, l3);
+                        st.put(q0 ? st.off(i0) : dis, ch.x, l0);
+                        st.put(q1 ? st.off(i1) : dis, ch.y, l1);
+                        st.put(q2 ? st.off(i2) : dis, ch.z, l2);
+                        st.put(q3 ? st.off(i3) : dis, ch.w, l3);
                     }
                 }
                 else
@@ -369,8 +377,6 @@ struct Tracer
             // sit in different leaves never serialises several dependent
             // fetches in one step.  The leaf's remaining references stay in
             // `cur` and are tested in slot order by the following steps.
-            const uint32_t first = cur & MCRT_LEAF_REF_MASK;
-            const uint32_t count = ((cur >> 29) & 3u) + 1u;
             const float4 a = f4_of(w0), b = f4_of(w1), c = f4_of(w2), d = f4_of(w3);
             if (COUNT) cnt.refs++;
             float lo;
@@ -387,9 +393,10 @@ struct Tracer
                     if (ANY) return true;
                 }
             }
-            if (count > 1)
+            if (cur >= (MCRT_CHILD_LEAF | (1u << 29)))
             {
-                cur = MCRT_CHILD_LEAF | ((count - 2u) << 29) | (first + 1u);
+                // more references follow: one fewer, the next one
+                cur = cur - (1u << 29) + 1u;
                 return false;
             }
         }

@@ -84,7 +84,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 
     Tracer<ANY, COUNT> tr;
     SceneRoot root;
     root.nodes = S.nodes;
-    root.refs = S.refs;
     root.det_eps = S.det_eps;
     root.node_span = S.node_span;
     uint32_t i = 0;

@@ -347,6 +347,18 @@ float qnode_span(const std::vector<QNode>& q)
     return f;
 }
 
+bool device_records(const WideBVH& wb, std::vector<QNode>& nodes_out)
+{
+    const uint64_t nq = wb.qnodes.size();
+    if (nq + wb.refs.size() > (uint64_t)MCRT_LEAF_REF_MASK + 1u) return false;
+    nodes_out = wb.qnodes;
+    for (QNode& q : nodes_out)
+        for (int c = 0; c < 4; c++)
+            if (q.child[c] != MCRT_CHILD_EMPTY && (q.child[c] & MCRT_CHILD_LEAF))
+                q.child[c] += (uint32_t)nq;  // stays below bit 29: every reference index + nq fits
+    return true;
+}
+
 void quantize_nodes(WideBVH& out)
 {
     out.qnodes.resize(out.nodes.size());

@@ -26,4 +26,10 @@ void quantize_nodes(WideBVH& out);
 // the scale of the rounding-error margin of the kernels' node tests
 // (trace_core.h: node_margin).  +inf if a node has non-finite planes.
 float qnode_span(const std::vector<QNode>& q);
+
+// The device's record array, nodes then references (layout.h): the QNodes
+// with every leaf link rebased by the node count.  Writes the nodes into
+// `nodes_out` (the references follow them unchanged).  Returns false when
+// the records do not fit the 29-bit link index.
+bool device_records(const WideBVH& wb, std::vector<QNode>& nodes_out);
 }  // namespace mcrt

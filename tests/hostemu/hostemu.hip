@@ -4,6 +4,7 @@
 // checked against the oracle on a machine without a GPU.  Built into
 // tests/hostemu/libhostemu.so; loaded only by tests.
 #include <algorithm>
+#include <cstring>
 #include <atomic>
 #include <thread>
 #include <vector>
@@ -20,6 +21,7 @@ struct Emu
 {
     std::vector<Tri> tris;
     WideBVH wb;
+    std::vector<QNode> recs;  // the device's record array: nodes, then references (layout.h)
     uint64_t digest = 0;
     long long stats[8] = {0};
     float det_eps = 1e-6f;  // Moller-Trumbore determinant cut of the traversal
@@ -44,6 +46,15 @@ extern "C" void* emu_build(const float* v9, long long n, int threads)
     for (const BNode& b : bt.nodes)
         if (b.leaf) { leaves++; if (b.ref_count == 0) empty++; }
     collapse_wide(bt, e->tris, e->wb);
+    if (!device_records(e->wb, e->recs))
+    {
+        delete e;
+        return nullptr;
+    }
+    const size_t nq = e->recs.size();
+    e->recs.resize(nq + e->wb.refs.size());
+    static_assert(sizeof(QNode) == sizeof(WRef), "records are 64 B");
+    if (!e->wb.refs.empty()) std::memcpy(&e->recs[nq], e->wb.refs.data(), e->wb.refs.size() * sizeof(WRef));
     e->stats[0] = (long long)bt.nodes.size();
     e->stats[1] = leaves;
     e->stats[2] = (long long)bt.refs.size();
@@ -122,8 +133,7 @@ extern "C" void emu_trace(void* h, const float* rays, long long n, int* hits, un
 {
     Emu* e = (Emu*)h;
     DevScene S;
-    S.nodes = e->wb.qnodes.empty() ? nullptr : e->wb.qnodes.data();
-    S.refs = e->wb.refs.data();
+    S.nodes = e->wb.qnodes.empty() ? nullptr : e->recs.data();
     S.stack_bound = e->wb.stack_bound;
     S.node_span = qnode_span(e->wb.qnodes);
     S.det_eps = e->det_eps;
