@@ -283,9 +283,12 @@ MCRT_API int mcrt_film_tonemap_gamma(float* film, size_t n, float gamma);
  * flight. */
 typedef struct mcrt_tuning {
     int32_t trace_waves_per_cu; /* traversal workgroups (one wavefront each) per CU,
-                                   1..1024 (28); the waves claim rays from work
-                                   counters, so more than fit at once only adds
-                                   launch overhead                                   */
+                                   1..1024; unset, finalize picks 32 (all resident)
+                                   when the BVH fits in 192 MB, else 28.  The waves
+                                   claim rays from work counters, so more than fit
+                                   at once only adds launch overhead; any-hit
+                                   queries use at most 28, what their kernel keeps
+                                   resident                                          */
     int32_t trace_lds_depth;    /* traversal stack entries per lane kept in LDS: 8, 12,
                                    16 or 24 (8); deeper entries go to a per-stream
                                    global spill area                                    */
@@ -293,7 +296,8 @@ typedef struct mcrt_tuning {
                                    (2^26); path state is ~0.5 KB per path              */
     int32_t pt_trace_waves_per_cu; /* the same two settings for the traversal
                                       launches of renders, whose ray sets (many
-                                      short shadow rays) favour more waves: (28) */
+                                      short shadow rays) favour more waves: (as
+                                      trace_waves_per_cu)                        */
     int32_t pt_trace_lds_depth;    /* (8)                                         */
     int64_t trace_reorder_rays;    /* a query of at least this many rays is
                                       traversed in the order of its rays' origin
@@ -314,13 +318,13 @@ MCRT_API int mcrt_scene_set_tuning(mcrt_scene* scene, const mcrt_tuning* tuning)
 MCRT_API int mcrt_scene_get_tuning(const mcrt_scene* scene, mcrt_tuning* tuning);
 
 /* Autotuner (no reference counterpart): times closest-hit over the n rays
- * for trace_waves_per_cu in {8,16,24,28,64,128} x trace_lds_depth in
+ * for trace_waves_per_cu in {8,16,24,28,32,64,128} x trace_lds_depth in
  * {8,12,16} and keeps the fastest, then trace_refill_lanes in {1,8,16,32}
  * with the chosen pair, then trace_any_refill_lanes in {16,32,48} (any-hit
  * over the same rays), then trace_reorder_rays -1 against min(n, 2^20)
  * (the caller's ray order against origin-cell order), keeping the fastest
  * each time; with `render` != NULL also times one render per
- * pt_trace_waves_per_cu in {16,28,64} x pt_trace_lds_depth in {8,16}, then
+ * pt_trace_waves_per_cu in {16,28,32,64} x pt_trace_lds_depth in {8,16}, then
  * per pt_trace_refill_lanes in {1,8,16,32}, then per pt_chunk_paths in
  * 2^22..2^26, keeping the fastest.  The chosen tuning is applied to the
  * scene and returned in `best`, the best closest-hit rate in
@@ -331,7 +335,7 @@ MCRT_API int mcrt_autotune(mcrt_scene* scene, const float* rays, size_t n,
                            mcrt_tuning* best, double* best_mrays_per_s);
 /* Autotuner of a batch (no reference counterpart): the render half of
  * mcrt_autotune timed on the whole batch -- pt_trace_waves_per_cu in
- * {16,28,64} x pt_trace_lds_depth in {8,16}, then pt_trace_refill_lanes in
+ * {16,28,32,64} x pt_trace_lds_depth in {8,16}, then pt_trace_refill_lanes in
  * {1,8,16,32}, then pt_chunk_paths in 2^22..2^26, one batch render per
  * setting into the batch's films, keeping the fastest each time.  The
  * chosen tuning is applied to scenes[0] (whose tuning the batch uses) and

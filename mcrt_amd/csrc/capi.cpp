@@ -606,7 +606,7 @@ static int autotune(mcrt_scene* s, const float* rays, size_t n, const mcrt_rende
             r = d_rays.as<float>();
         }
         if ((rc = d_hits.ensure(n * sizeof(mcrt_hit)))) return rc;
-        static const int kWaves[] = {8, 16, 24, 28, 64, 128};
+        static const int kWaves[] = {8, 16, 24, 28, 32, 64, 128};
         static const int kDepth[] = {8, 12, 16};
         int bw = s->trace_waves_per_cu, bd = s->trace_lds_depth;
         for (int w : kWaves)
@@ -731,7 +731,7 @@ static int autotune(mcrt_scene* s, const float* rays, size_t n, const mcrt_rende
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             return MCRT_OK;
         };
-        static const int kPtWaves[] = {16, 28, 64};
+        static const int kPtWaves[] = {16, 28, 32, 64};
         static const int kPtDepth[] = {8, 16};
         int pw = s->pt_trace_waves_per_cu, pd = s->pt_trace_lds_depth;
         float best_ms = 0.0f;
@@ -893,7 +893,7 @@ static int batch_autotune(mcrt_batch* b, hipStream_t st, double* best_ms_out)
     };
     float ms = 0.0f, best_ms = 0.0f;
     if ((rc = time_batch(ms))) return rc;  // warm-up: code objects, workspace
-    static const int kPtWaves[] = {16, 28, 64};
+    static const int kPtWaves[] = {16, 28, 32, 64};
     static const int kPtDepth[] = {8, 16};
     int pw = s0->pt_trace_waves_per_cu, pd = s0->pt_trace_lds_depth;
     for (int w : kPtWaves)

@@ -139,11 +139,12 @@ struct mcrt_scene
     // defaults measured on MI355X (1.3M-triangle displaced icosphere, 16M
     // incoherent rays; 1024^2 renders of it and of an outdoor scene): waves
     // claim rays from work counters, so the grid only has to fill the
-    // machine once: 28 one-wave workgroups per CU with 8-entry LDS stacks
-    // (7 waves per SIMD, all resident) were fastest for queries and renders alike
-    int32_t trace_waves_per_cu = 28;
+    // machine once, with 8-entry LDS stacks.  Waves per CU: 0 = not set,
+    // decided at finalize by the BVH's size (32, all resident, over a
+    // cache-resident BVH; 28 over a larger one)
+    int32_t trace_waves_per_cu = 0;
     int32_t trace_lds_depth = 8;
-    int32_t pt_trace_waves_per_cu = 28;
+    int32_t pt_trace_waves_per_cu = 0;
     int32_t pt_trace_lds_depth = 8;
     // 2^26 paths per chunk (a whole 1024^2 x 64-spp frame; about 29 GB of
     // path state, 43 GB with an environment map): fewer, larger launches

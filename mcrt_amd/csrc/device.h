@@ -49,6 +49,8 @@ struct DevScene
     int64_t ntri = 0;
     uint64_t* spill = nullptr;         // traversal stack overflow storage
     int64_t spill_lanes = 0;           // lanes the spill buffer covers = resident lanes of a launch
+    int64_t lanes_any = 0;             // lanes of an any-hit launch: at most spill_lanes, and at most
+                                       // what the any-hit kernel keeps resident (kTraceAnyWavesPerSimd)
     int32_t spill_depth = 0;           // entries per lane beyond the LDS stack
     int32_t lds_depth = 16;            // LDS stack entries per lane (8, 12, 16 or 24)
     uint32_t* wq = nullptr;            // ray-query work counters, kWqWords words (per stream)
@@ -84,6 +86,12 @@ struct SceneRoot
     float det_eps = 1e-6f;  // the scene's Moller-Trumbore determinant cut
     float node_span = 0.0f; // qnode_span of the scene's nodes
 };
+
+// Waves per SIMD the traversal kernels with an 8-entry LDS stack are
+// compiled to keep resident (four SIMDs per CU): closest hit 8 (64 VGPRs),
+// any hit 7 (held to eight waves' registers it spills scalar registers).
+constexpr int kTraceWavesPerSimd = 8;
+constexpr int kTraceAnyWavesPerSimd = 7;
 
 // Work counters of a traversal launch: one per slice of the rays, 64 B
 // apart (trace_kernel.h).

@@ -481,11 +481,15 @@ int finalize_scene(mcrt_scene* s, int threads, const char* cache)
     // 8.5 instead of 10.7 ms, sort included).  Over a cache-resident BVH the
     // sort costs more than it saves (the 101 MB one of the 1.3M-triangle
     // mesh: 2.67 instead of 2.08 ms).
-    if (s->trace_reorder_rays == 0)
-    {
-        const size_t bvh = s->d_bvh.bytes;
-        s->trace_reorder_rays = bvh > (size_t(192) << 20) ? int64_t(1) << 20 : -1;
-    }
+    // Unless set: 32 traversal waves per CU (8 per SIMD, all resident) over
+    // a cache-resident BVH, 28 (7 per SIMD) over a larger one, where more
+    // rays in flight evict more of each other's nodes (the 1.3M-triangle
+    // mesh's queries 1.5% and its frame 2% faster at 32; the soup's
+    // incoherent closest-hit queries 3% slower).
+    const bool big_bvh = s->d_bvh.bytes > (size_t(192) << 20);
+    if (s->trace_reorder_rays == 0) s->trace_reorder_rays = big_bvh ? int64_t(1) << 20 : -1;
+    if (s->trace_waves_per_cu == 0) s->trace_waves_per_cu = big_bvh ? 28 : 4 * kTraceWavesPerSimd;
+    if (s->pt_trace_waves_per_cu == 0) s->pt_trace_waves_per_cu = big_bvh ? 28 : 4 * kTraceWavesPerSimd;
     apply_trace_tuning(s);
     auto t2 = clk::now();
     st.build_seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -619,6 +623,8 @@ void trace_geometry(DevScene& d, int num_cus, int waves_per_cu, int lds_depth)
 {
     d.lds_depth = lds_depth;
     d.spill_lanes = (int64_t)num_cus * waves_per_cu * 64;
+    const int any_waves = 4 * kTraceAnyWavesPerSimd;
+    d.lanes_any = (int64_t)num_cus * (waves_per_cu < any_waves ? waves_per_cu : any_waves) * 64;
     d.spill_depth = d.stack_bound > d.lds_depth ? d.stack_bound - d.lds_depth : 0;
     d.spill = nullptr;  // per stream, see launch_view
 }

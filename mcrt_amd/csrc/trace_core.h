@@ -41,6 +41,9 @@ struct RayPrep
     float ix, iy, iz;
     float R0;
     float w;     // node-test margin scale (node_margin)
+    // per axis, -RN(o * inv) -/+ RN(w * |inv|): the node-test plane map's
+    // offsets before the node origin's term (node_margin)
+    float cnx, cny, cnz, cfx, cfy, cfz;
     bool sx, sy, sz;
     bool exact;  // an inv_dir component is infinite (or the node-test
                  // margins would leave their bounds): use the reference's
@@ -62,6 +65,7 @@ MCRT_HD RayPrep prep_ray(float ox, float oy, float oz, float dx, float dy, float
     r.sx = dx < 0.0f; r.sy = dy < 0.0f; r.sz = dz < 0.0f;
     r.R0 = tmin;
     r.w = 0.0f;
+    r.cnx = r.cny = r.cnz = r.cfx = r.cfy = r.cfz = 0.0f;
     r.exact = __builtin_isinf(r.ix) || __builtin_isinf(r.iy) || __builtin_isinf(r.iz);
     // octant (dx > 0) | (dy > 0) << 1 | (dz > 0) << 2 of the nodes' rank tables
     r.oct_hi = dz > 0.0f;
@@ -109,19 +113,25 @@ MCRT_HD bool slab(const RayPrep& r, float lx, float hx, float ly, float hy, floa
 // as it passes wherever the reference's slab arithmetic on the decoded box,
 // t = RN(RN(plane - o) * inv), passes.  With plane = RN(origin + q * s) it
 // maps the plane code q straight to
-//     t_near = fma(q, s * inv, (origin - o) * inv - w * |inv|)
-//     t_far  = fma(q, s * inv, (origin - o) * inv + w * |inv|),
-// two instructions per plane instead of four (s * inv is exact: s = 2^k).
-// Against the reference arithmetic its rounding error is below
-// u * |inv| * (5 |origin - o| + |plane| + 510 s) (u = 2^-24), plus
-// subnormal terms below 2^-140.  The margin w * |inv| covers both when
-// w = 2^-21 (max |o| + 2 span), span >= |origin| + 255 s of every node
-// (qnode_span), and w * |inv| >= 2^-100 for every non-zero inv.  So t_near
-// is at most, and t_far at least, the reference value, and the node test
-// passes wherever the reference-arithmetic test passes.  With inv = 0 both
-// are exactly the reference's 0.  Every quantity stays finite while
-// w * |inv| < 2^100.  A ray outside these bounds is marked exact: it
-// descends every child and is decided by the exact reference tests.
+//     t_near = fma(q, s * inv, fma(origin, inv, c_near))
+//     t_far  = fma(q, s * inv, fma(origin, inv, c_far)),
+// c_near / c_far = RN(-RN(o * inv) -/+ m), m = RN(w * |inv|), formed once per
+// ray: per node one fused multiply-add per axis and side, and per plane one
+// conversion and one fused multiply-add (s * inv is exact: s = 2^k), where
+// the reference arithmetic takes four.  Against it the rounding error of
+// t_near is below u * |inv| * (6 max |o| + 5 span) + 3 u m to first order
+// (u = 2^-24; with span >= |origin| + 255 s, hence >= |plane|), plus
+// subnormal terms below 2^-146: the four roundings here are bounded by
+// u (|o inv| + |o inv| + m + |origin - o| |inv| + m + |plane - o| |inv| + m),
+// the reference's three by u |inv| (|plane| + 2 |plane - o|).  The margin m
+// covers that with room when w = 2^-21 (max |o| + 2 span) = 8 u (...),
+// span = qnode_span over the nodes, and w * |inv| >= 2^-100 for every
+// non-zero inv.  So t_near is at most, and t_far at least, the reference
+// value, and the node test passes wherever the reference-arithmetic test
+// passes.  With inv = 0 both are exactly the reference's 0.  Every quantity
+// stays finite while w * |inv| < 2^100.  A ray outside these bounds is
+// marked exact: it descends every child and is decided by the exact
+// reference tests.
 MCRT_HD void node_margin(RayPrep& r, float span)
 {
     const float om = fmaxf(fmaxf(fabsf(r.ox), fabsf(r.oy)), fabsf(r.oz));
@@ -132,6 +142,13 @@ MCRT_HD void node_margin(RayPrep& r, float span)
     const bool okz = r.iz == 0.0f || (pz >= 0x1p-100f && pz < 0x1p100f);
     r.w = w;
     r.exact = r.exact || !(okx && oky && okz);
+    const float qx = r.ox * r.ix, qy = r.oy * r.iy, qz = r.oz * r.iz;
+    r.cnx = -qx - px;
+    r.cfx = px - qx;
+    r.cny = -qy - py;
+    r.cfy = py - qy;
+    r.cnz = -qz - pz;
+    r.cfz = pz - qz;
 }
 
 // Conservative test of child `shift / 8` of a QNode (node_margin), given the
@@ -298,11 +315,10 @@ struct Tracer
                 const uint32_t ny = r.sy ? qa.w : qa.z, fy = r.sy ? qa.z : qa.w;
                 const uint32_t nz = r.sz ? qb.y : qb.x, fz = r.sz ? qb.x : qb.y;
                 // the per-axis plane maps (node_margin), shared by the four children
-                const float bx = (ox - r.ox) * r.ix, by = (oy - r.oy) * r.iy, bz = (oz - r.oz) * r.iz;
                 const float ax = sx * r.ix, ay = sy * r.iy, az = sz * r.iz;
-                const float nbx = __builtin_fmaf(-r.w, fabsf(r.ix), bx), fbx = __builtin_fmaf(r.w, fabsf(r.ix), bx);
-                const float nby = __builtin_fmaf(-r.w, fabsf(r.iy), by), fby = __builtin_fmaf(r.w, fabsf(r.iy), by);
-                const float nbz = __builtin_fmaf(-r.w, fabsf(r.iz), bz), fbz = __builtin_fmaf(r.w, fabsf(r.iz), bz);
+                const float nbx = __builtin_fmaf(ox, r.ix, r.cnx), fbx = __builtin_fmaf(ox, r.ix, r.cfx);
+                const float nby = __builtin_fmaf(oy, r.iy, r.cny), fby = __builtin_fmaf(oy, r.iy, r.cfy);
+                const float nbz = __builtin_fmaf(oz, r.iz, r.cnz), fbz = __builtin_fmaf(oz, r.iz, r.cfz);
                 p0 = qslab_c(r, ch.x, nx, fx, ny, fy, nz, fz, ax, ay, az, nbx, fbx, nby, fby, nbz, fbz, 0, R1, l0);
                 p1 = qslab_c(r, ch.y, nx, fx, ny, fy, nz, fz, ax, ay, az, nbx, fbx, nby, fby, nbz, fbz, 8, R1, l1);
                 p2 = qslab_c(r, ch.z, nx, fx, ny, fy, nz, fz, ax, ay, az, nbx, fbx, nby, fby, nbz, fbz, 16, R1, l2);

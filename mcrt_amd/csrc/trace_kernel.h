@@ -44,14 +44,15 @@ struct TraceBatch
     const uint32_t* skip = nullptr;
 };
 
-// With an 8-entry stack the kernel is compiled for at least seven waves per
-// SIMD (28 per CU, with LDS for all of them; the closest-hit kernel takes 59
-// VGPRs).  Held to eight waves' budget the compiler spilled SGPRs into VGPR
-// lanes, and the spill code in the refill block cost more than the eighth
-// wave gained.
-// Deeper stacks are held to fewer waves by LDS.
+// With an 8-entry stack the closest-hit kernel is compiled for eight waves
+// per SIMD (32 per CU, with LDS for all of them: 64 VGPRs), the any-hit
+// kernel for seven (held to eight waves' registers it spills scalar
+// registers into vector lanes, and the spill code in the refill block costs
+// more than the eighth wave gains).  Deeper stacks are held to fewer waves
+// by LDS.
 template <int D, bool ANY, bool COUNT, bool SEG, bool BATCH, bool ORDER>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(D <= 8 ? 7 : 4, 8))) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
+    D <= 8 ? (ANY ? kTraceAnyWavesPerSimd : kTraceWavesPerSimd) : 4, 8))) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
                                                  const uint32_t* __restrict__ seg_cnt, int64_t seg1,
                                                  int4* __restrict__ hits, uint8_t* __restrict__ occ,
                                                  unsigned long long* __restrict__ counters, TraceBatch B,

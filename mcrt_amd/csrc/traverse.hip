@@ -218,9 +218,15 @@ hipError_t launch_trace(const DevScene& s, const float* rays, int64_t n, void* h
 {
     const bool any = mode == TRACE_ANY;
     // any-hit queries have their own compaction threshold (their rays end at
-    // the first hit, so lanes free up faster)
+    // the first hit, so lanes free up faster), and a grid of at most the
+    // waves their kernel keeps resident (waves claim rays, so workgroups
+    // beyond those only start late)
     DevScene sa = s;
-    if (any) sa.refill_lanes = s.refill_lanes_any;
+    if (any)
+    {
+        sa.refill_lanes = s.refill_lanes_any;
+        if (s.lanes_any < sa.spill_lanes) sa.spill_lanes = s.lanes_any;
+    }
     const int64_t kMaxLaunch = int64_t(1) << 30;  // the kernel indexes rays with 32 bits
     for (int64_t off = 0; off < n; off += kMaxLaunch)
     {

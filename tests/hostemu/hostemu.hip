@@ -199,8 +199,9 @@ extern "C" void emu_trace(void* h, const float* rays, long long n, int* hits, un
 // (the tightest span a scene can give it).  out[0] = cases checked (rays
 // the margin's bounds admit), out[1] = cases where the lower bound exceeds
 // or the upper bound falls below RN(RN(plane - o) * inv), out[2] = cases
-// sent to the exact path; worst = the largest |t_ref - (q A + B)| / (w |inv|)
-// seen (the margin's share the rounding used).  `wscale` multiplies the
+// sent to the exact path; worst = the largest share of the margin m the
+// rounding used: (t_near - (t_ref - m)) / m or ((t_ref + m) - t_far) / m
+// (1 or more: a bound crossed t_ref).  `wscale` multiplies the
 // margin (1 for the product's; below 1 to show the check catches a margin
 // that is too small).
 extern "C" void emu_margin_check(long long n, unsigned long long seed, float wscale, long long* out, double* worst)
@@ -239,7 +240,14 @@ extern "C" void emu_margin_check(long long n, unsigned long long seed, float wsc
         }
         RayPrep r = prep_ray(o[0], o[1], o[2], d[0], d[1], d[2], 1e-4f);
         node_margin(r, span);
-        r.w *= wscale;
+        if (wscale != 1.0f)
+        {
+            // the x axis's offsets re-formed with the margin scaled
+            r.w *= wscale;
+            const float m = r.w * std::fabs(r.ix), p = r.ox * r.ix;
+            r.cnx = -p - m;
+            r.cfx = m - p;
+        }
         if (r.exact)
         {
             exact++;
@@ -249,16 +257,16 @@ extern "C" void emu_margin_check(long long n, unsigned long long seed, float wsc
         const float P = __builtin_fmaf((float)q, s, O);
         const float tref = (P - oo) * inv;
         const float A = s * inv;
-        const float B = (O - oo) * inv;
-        const float tn = __builtin_fmaf((float)q, A, __builtin_fmaf(-r.w, std::fabs(inv), B));
-        const float tf = __builtin_fmaf((float)q, A, __builtin_fmaf(r.w, std::fabs(inv), B));
+        // the kernels' node-step plane map (trace_core.h: Tracer::visit)
+        const float tn = __builtin_fmaf((float)q, A, __builtin_fmaf(O, inv, r.cnx));
+        const float tf = __builtin_fmaf((float)q, A, __builtin_fmaf(O, inv, r.cfx));
         checked++;
         if (!(tn <= tref && tf >= tref)) bad++;
         const double m = (double)r.w * std::fabs((double)inv);
         if (m > 0.0)
         {
-            const double e = std::fabs((double)tref - ((double)q * (double)A + (double)B)) / m;
-            if (e > wmax) wmax = e;
+            const double en = ((double)tn - ((double)tref - m)) / m, ef = (((double)tref + m) - (double)tf) / m;
+            wmax = std::max(wmax, std::max(en, ef));
         }
     }
     out[0] = checked;

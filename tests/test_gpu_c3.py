@@ -42,6 +42,9 @@ def test_c3_soup_full_size_bit_exact(ctx):
     print(f"[c3] build {st['build_seconds']:.1f} s refs {st['refs']} stack_bound {st['stack_bound']}", flush=True)
     assert str(st["tree_digest"]) == G["tree_digest"]
     assert st["refs"] == G["tree_stats"]["refs"]
+    # unset, a BVH larger than the cache gets 28 waves per CU and ordered queries
+    t = s.tuning()
+    assert (t["trace_waves_per_cu"], t["pt_trace_waves_per_cu"], t["trace_reorder_rays"]) == (28, 28, 1 << 20)
     for kind, g in G["sets"].items():
         print(f"[c3] {kind}", flush=True)
         rays = scenes.c3_rays(kind)

@@ -30,7 +30,10 @@ def test_traversal_tuning_bit_exact(ctx, depth, waves):
     print(f"[tuning] depth={depth} waves={waves} refill={refill} any_refill={any_refill}", flush=True)
     tris, s, st = _debug_scene(ctx)
     assert st["stack_bound"] > 16, "fixture spills at LDS depths 8, 12 and 16"
-    assert s.tuning()["trace_any_refill_lanes"] == 32
+    t0 = s.tuning()
+    assert t0["trace_any_refill_lanes"] == 32
+    # unset, a BVH that fits the cache gets 32 waves per CU (8 per SIMD)
+    assert (t0["trace_waves_per_cu"], t0["pt_trace_waves_per_cu"]) == (32, 32)
     s.set_tuning(trace_waves_per_cu=waves, trace_lds_depth=depth, trace_refill_lanes=refill,
                  trace_any_refill_lanes=any_refill)
     t = s.tuning()
@@ -126,9 +129,9 @@ def test_autotune_applies_valid_tuning(ctx):
     print(f"[autotune] {t} {rate:.1f} Mrays/s", flush=True)
     assert t == s.tuning()
     assert t["trace_lds_depth"] in (8, 12, 16)
-    assert t["trace_waves_per_cu"] in (8, 16, 24, 28, 64, 128)
+    assert t["trace_waves_per_cu"] in (8, 16, 24, 28, 32, 64, 128)
     assert t["pt_chunk_paths"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
-    assert t["pt_trace_waves_per_cu"] in (16, 28, 64) and t["pt_trace_lds_depth"] in (8, 16)
+    assert t["pt_trace_waves_per_cu"] in (16, 28, 32, 64) and t["pt_trace_lds_depth"] in (8, 16)
     assert t["trace_refill_lanes"] in (1, 8, 16, 32) and t["pt_trace_refill_lanes"] in (1, 8, 16, 32)
     assert t["trace_any_refill_lanes"] in (16, 32, 48)
     assert t["trace_reorder_rays"] in (-1, rays.shape[0])
