@@ -334,7 +334,8 @@ class Scene:
                    pt_trace_lds_depth=0, trace_reorder_rays=0, trace_refill_lanes=0, pt_trace_refill_lanes=0,
                    trace_any_refill_lanes=0):
         """Launch tuning (0 keeps a field; trace_reorder_rays=-1: never
-        reorder queries); results never depend on it."""
+        reorder queries); results never depend on it.  Fields left unset
+        before finalize are picked there by the BVH's size."""
         t = Tuning(trace_waves_per_cu, trace_lds_depth, pt_chunk_paths, pt_trace_waves_per_cu, pt_trace_lds_depth,
                    trace_reorder_rays, trace_refill_lanes, pt_trace_refill_lanes, trace_any_refill_lanes)
         _check(lib().mcrt_scene_set_tuning(self.h, ctypes.byref(t)))
