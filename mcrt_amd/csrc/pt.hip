@@ -20,12 +20,15 @@
 // Every kernel is grid-strided over a fixed grid and reads its work count
 // from device memory, so the whole chunk is launch-only (graph-capturable).
 //
-// A diffuse vertex's next-event record holds rs rays: the BSDF-sampled ray
+// A diffuse vertex's next-event record has rs rays: the BSDF-sampled ray
 // and the light-sampled shadow ray of the area lights, and, with an
 // environment map (rs = 4), the shadow rays of its two environment samples
 // (one drawn from the map's pdf, one from the cosine lobe; demo_pt.cpp:
-// 421-499, 564-589).  A ray a scene does not need has a zero direction,
-// which every box rejects.
+// 421-499, 564-589).  The record's rays sit at its slots, except that
+// without an environment map the BSDF-sampled ray is traced only where it
+// can reach an emitter and then goes to a compacted segment of its own (its
+// slot kept in the record).  A ray a scene does not need has a zero
+// direction, which every box rejects.
 //
 // Batched rendering (BATCH = true): the paths of many scenes -- each with
 // its own BVH, camera, film size, spp and depth -- share the chunks.  A
@@ -64,11 +67,14 @@ struct PTState
     float4* T;           // throughput (xyz)
     float4* L;           // radiance (xyz)
     uint8_t* last_diff;  // previous vertex was diffuse
-    float4* rays[2];     // [P continuation | rs P next-event] x 2 float4 per ray
+    float4* rays[2];     // [P continuation | rec_rays(rs) P next-event at the records' slots |
+                         //  P BSDF-sampled next-event, compacted (rs = 2)] x 2 float4 per ray
     uint32_t* act[2];    // path id per continuation ray
     float4* rec[2];      // next-event records, rec_words(rs) float4 each
-    uint32_t* cnt[2];    // {continuation rays, next-event rays}, then the traversal's work
-                         // counters from word kCntWq on (kSideWords words per side)
+    uint32_t* cnt[2];    // {continuation rays, next-event rays at record slots}, the
+                         // compacted BSDF-sampled rays at word kSeg2Word, then the
+                         // traversal's work counters from word kCntWq on (kSideWords
+                         // words per side)
     int4* hits;          // (1 + rs) P
     float4* samples;     // P clamped sample radiance
     float4* Ltot;        // adjoint: radiance of pass 1
@@ -82,11 +88,14 @@ struct PTState
 };
 
 // float4 words of a next-event record: {T, pdf_b}, {albedo, factor2},
-// {path, light, -, -}; with the environment {T, pdf_b}, {albedo, factor2},
-// {e1, path}, {e2, light}.  The directions of its rays are read back from
-// the rays themselves (the resolve step needs them only where a ray hit an
-// emitter).
+// {path, light, BSDF ray slot or -1, -}; with the environment {T, pdf_b},
+// {albedo, factor2}, {e1, path}, {e2, light}.  The directions of its rays
+// are read back from the rays themselves (the resolve step needs them only
+// where a ray hit an emitter).
 __host__ __device__ constexpr int rec_words(int rs) { return rs == 4 ? 4 : 3; }
+// Rays a record keeps at its own slots: the light sample's shadow ray (the
+// BSDF-sampled ray is compacted), or with an environment map all four.
+__host__ __device__ constexpr int rec_rays(int rs) { return rs == 4 ? 4 : 1; }
 
 // One scene of a render: its device data, camera and parameters.
 struct ScenePT
@@ -140,39 +149,52 @@ __device__ __forceinline__ V3f f3(float4 v) { return V3f{v.x, v.y, v.z}; }
 // coalesce.  Every thread of the workgroup calls it (it synchronises the
 // workgroup).  Slot order never affects a result: every path's records
 // are its own.
-__device__ __forceinline__ void block_slots(uint32_t* c, bool a, bool b, uint32_t per, uint32_t& sa, uint32_t& sb)
+// With E, a third compaction (counter c[kSeg2Word], on its own cache line,
+// one slot each, slot se) by a second atomic issued beside the first.
+template <bool E>
+__device__ __forceinline__ void block_slots(uint32_t* c, bool a, bool b, bool e, uint32_t per, uint32_t& sa,
+                                            uint32_t& sb, uint32_t& se)
 {
     constexpr uint32_t W = kTPB / 64;
-    __shared__ uint32_t s_n[2][W];
+    __shared__ uint32_t s_n[3][W];
     __shared__ unsigned long long s_base;
-    const uint64_t ma = __ballot(a), mb = __ballot(b);
+    __shared__ uint32_t s_base_e;
+    const uint64_t ma = __ballot(a), mb = __ballot(b), me = E ? __ballot(e) : 0ull;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     if (lane == 0)
     {
         s_n[0][w] = (uint32_t)__popcll(ma);
         s_n[1][w] = (uint32_t)__popcll(mb);
+        if (E) s_n[2][w] = (uint32_t)__popcll(me);
     }
     __syncthreads();
-    uint32_t pa = 0, pb = 0, ta = 0, tb = 0;
+    uint32_t pa = 0, pb = 0, pe = 0, ta = 0, tb = 0, te = 0;
     for (uint32_t q = 0; q < W; q++)
     {
-        const uint32_t na = s_n[0][q], nb = s_n[1][q];
+        const uint32_t na = s_n[0][q], nb = s_n[1][q], ne = E ? s_n[2][q] : 0u;
         pa += q < w ? na : 0u;
         pb += q < w ? nb : 0u;
+        pe += q < w ? ne : 0u;
         ta += na;
         tb += nb;
+        te += ne;
     }
     // c[0] stays below 2^28 (the largest chunk), so the low word never
     // carries into c[1]
-    if (threadIdx.x == 0 && (ta | tb) != 0)
-        s_base = atomicAdd((unsigned long long*)c, (unsigned long long)ta | ((unsigned long long)(tb * per) << 32));
+    if (threadIdx.x == 0)
+    {
+        if ((ta | tb) != 0)
+            s_base = atomicAdd((unsigned long long*)c, (unsigned long long)ta | ((unsigned long long)(tb * per) << 32));
+        if (E && te != 0) s_base_e = atomicAdd(c + kSeg2Word, te);
+    }
     __syncthreads();
-    // s_n and s_base are rewritten only after the next call's first barrier,
-    // which every thread reaches after these reads
+    // s_n and the bases are rewritten only after the next call's first
+    // barrier, which every thread reaches after these reads
     const unsigned long long base = s_base;
     const uint64_t below = (1ull << lane) - 1ull;
     sa = (uint32_t)base + pa + (uint32_t)__popcll(ma & below);
     sb = (uint32_t)(base >> 32) + (pb + (uint32_t)__popcll(mb & below)) * per;
+    se = E ? s_base_e + pe + (uint32_t)__popcll(me & below) : 0u;
 }
 __device__ __forceinline__ float4 f4(V3f v) { return make_float4(v.x, v.y, v.z, 0.0f); }
 
@@ -255,7 +277,8 @@ __device__ __forceinline__ void grad_add(float* acc, float* g, int key, int hot,
 }
 
 // Counts of one side, and its traversal work counters zeroed.
-constexpr int kCntWq = 16;
+constexpr int kCntWq = 64;
+static_assert(kSeg2Word >= 32 && kSeg2Word < kCntWq, "the third count sits on its own 128 B line before the work counters");
 constexpr int kSideWords = kCntWq + (int)kWqWords;
 __global__ void k_set_counts(uint32_t* cnt, uint32_t a, uint32_t b)
 {
@@ -264,6 +287,7 @@ __global__ void k_set_counts(uint32_t* cnt, uint32_t a, uint32_t b)
     {
         cnt[0] = a;
         cnt[1] = b;
+        cnt[kSeg2Word] = 0;
     }
 }
 
@@ -347,10 +371,11 @@ __global__ void k_generate(PTState S, PTArgs a)
 // then the light sample if its shadow ray reaches the sampled light.
 // em1/em2, hot1/hot2 and the coefficients c1/c2 of the two emissions are
 // what the adjoint scatters.
-// nr: the record's rays (BSDF-sampled ray, then the light sample's shadow
-// ray; 2 float4 each, the direction in the second).
+// rb / rl: the BSDF-sampled ray and the light sample's shadow ray (2 float4
+// each, the direction in the second; read only where the ray hit).
 __device__ __forceinline__ V3f area_terms(const DevScene& D, const int4 h1, const int4 h2, V3f col,
-                                          const float4* nr, float pdf_b, float factor2, int li, V3f local, int& em1,
+                                          const float4* rb, const float4* rl, float pdf_b, float factor2, int li,
+                                          V3f local, int& em1,
                                           int& em2, int& hot1, int& hot2, V3f& c1, V3f& c2)
 {
     float pdf_l = 0.0f;
@@ -362,7 +387,7 @@ __device__ __forceinline__ V3f area_terms(const DevScene& D, const int4 h1, cons
         if (mt.type == MCRT_LGHT)
         {
             const V3f ns1 = shading_normal(ts, float_of((uint32_t)h1.z), float_of((uint32_t)h1.w));
-            const V3f wb = f3(nr[1]);
+            const V3f wb = f3(rb[1]);
             const float cosl = dot(ns1, -wb);
             if (!(cosl < 1e-6f))
             {
@@ -393,7 +418,7 @@ __device__ __forceinline__ V3f area_terms(const DevScene& D, const int4 h1, cons
             if (mt.type == MCRT_LGHT)
             {
                 const V3f ns2 = shading_normal(ts, float_of((uint32_t)h2.z), float_of((uint32_t)h2.w));
-                const V3f wo = f3(nr[3]);
+                const V3f wo = f3(rl[1]);
                 if (!(dot(ns2, -wo) < 0) && ts.light == li)
                 {
                     le2 = ld3(mt.rgb);
@@ -420,9 +445,10 @@ __global__ void k_resolve(PTArgs a, PTState S, int cur)
     __shared__ float acc[ADJ ? 2 * kHotFloats : 1];
     if (ADJ) hot_clear(acc);
     const int64_t P = a.P;
-    constexpr int rs = ENV ? 4 : 2;
-    const uint32_t nrec = S.cnt[cur][1] / (uint32_t)rs;
+    constexpr int rs = ENV ? 4 : 2, rf = rec_rays(rs);
+    const uint32_t nrec = S.cnt[cur][1] / (uint32_t)rf;
     const float4* R = S.rec[cur];
+    const float4* rays = S.rays[cur];
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nrec; base += (int64_t)gridDim.x * blockDim.x)
     {
         const int64_t m = base + threadIdx.x;
@@ -436,7 +462,7 @@ __global__ void k_resolve(PTArgs a, PTState S, int cur)
             const V3f T = f3(r0), col = f3(r1);
             const uint32_t path = bits_of(ENV ? r2.w : r2.x);
             const DevScene& D = BATCH ? a.tab[S.sid[path]].D : a.one.D;
-            const int64_t h0 = P + rs * m;
+            const int64_t h0 = P + rf * m;  // the record's first ray slot
             V3f local = v3f(0.0f, 0.0f, 0.0f);
             if (ENV)
             {
@@ -448,9 +474,16 @@ __global__ void k_resolve(PTArgs a, PTState S, int cur)
             if (D.nlights > 0)
             {
                 const int li = (int)bits_of(ENV ? r3.w : r2.y);
+                const int4 miss = make_int4(-1, 0, 0, 0);
+                // the BSDF-sampled ray: at the record's first slot, or (without
+                // an environment map) in the compacted segment after the
+                // records' slots where it was traced, else a miss
+                const int bs = ENV ? 0 : (int)bits_of(r2.z);
+                const int64_t hb = ENV ? h0 : P + rf * P + bs, hl = ENV ? h0 + 1 : h0;
                 V3f c1, c2;
-                local = area_terms(D, S.hits[h0], li >= 0 ? S.hits[h0 + 1] : make_int4(-1, 0, 0, 0), col,
-                                   S.rays[cur] + 2 * h0, r0.w, r1.w, li, local, em1, em2, hot1, hot2, c1, c2);
+                local = area_terms(D, (ENV || bs >= 0) ? S.hits[hb] : miss, li >= 0 ? S.hits[hl] : miss, col,
+                                   rays + 2 * hb, rays + 2 * hl, r0.w, r1.w, li, local, em1, em2, hot1, hot2, c1,
+                                   c2);
                 if (ADJ)
                 {
                     const V3f dl = f3(S.delta[path]);
@@ -511,6 +544,40 @@ __device__ __forceinline__ void env_terms(const DevScene& D, uint64_t key, V3f n
     }
 }
 
+// Of a vertex at hit h of ray j: the ray's direction d, the hit point, the
+// shading normal and, where it has a next-event record and the scene area
+// lights, the BSDF-sampled next-event direction wb and whether that ray is
+// traced.  The BSDF-sampled ray adds a term only when its closest hit is an
+// emitter, and the traversal accepts a triangle only after its reference's
+// box passes the slab test.  That test is monotone in the box
+// (round-to-nearest keeps (b - o) * inv_dir ordered in b), so a ray failing
+// it against the box around every emitter reference (with tmax = inf)
+// reaches none, and its terms are a miss's: it is not traced.  Rays with an
+// infinite inv_dir take the reference's NaN-prone sequential test and are
+// always traced.
+__device__ __forceinline__ void vertex_dirs(const DevScene& D, const float4* rays, int64_t j, int4 h, const TriShade& ts,
+                                            bool want_rec, uint64_t key, V3f& d, V3f& phit, V3f& ns, V3f& wb,
+                                            bool& trace_wb)
+{
+    const float4 ra = rays[2 * j], rb = rays[2 * j + 1];
+    const V3f o = f3(ra);
+    d = f3(rb);
+    const float t = float_of((uint32_t)h.y), u = float_of((uint32_t)h.z), v = float_of((uint32_t)h.w);
+    phit = o + d * t;
+    ns = shading_normal(ts, u, v);
+    if (want_rec && D.nlights > 0)
+    {
+        wb = lambert_dir(ns, rng_u01(key, 0), rng_u01(key, 1));
+        if (D.emit_any)
+        {
+            const RayPrep rw = prep_ray(phit.x, phit.y, phit.z, wb.x, wb.y, wb.z, kEpsRay);
+            float lo;
+            trace_wb = rw.exact || slab(rw, D.emit_lo[0], D.emit_hi[0], D.emit_lo[1], D.emit_hi[1], D.emit_lo[2],
+                                        D.emit_hi[2], __builtin_inff(), lo);
+        }
+    }
+}
+
 // One vertex of every live path (demo_pt.cpp:523-651).  ENV: a scene of the
 // launch has an environment map (escape radiance, rs = 4 next-event rays);
 // without one the environment code is compiled out, which keeps the
@@ -524,7 +591,7 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
 {
     __shared__ float acc[ADJ ? 2 * kHotFloats : 1];
     if (ADJ) hot_clear(acc);
-    constexpr int rs = ENV ? 4 : 2;
+    constexpr int rs = ENV ? 4 : 2, rf = rec_rays(rs);
     const int nxt = cur ^ 1;
     const uint32_t A = S.cnt[cur][0];
     const float4* rays = S.rays[cur];
@@ -568,8 +635,17 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 want_cont = !(k >= sp.max_depth) && !(k >= sp.rr_start && rng_u01(key, 7) >= sp.rr_prob);
             }
         }
-        uint32_t cslot, rslot;
-        block_slots(S.cnt[nxt], want_cont, want_rec, (uint32_t)rs, cslot, rslot);
+        // The hit point, the shading normal and the BSDF-sampled next-event
+        // direction: without an environment map they come before the
+        // workgroup takes its slots, since whether that ray is traced decides
+        // whether it takes a slot of the compacted segment; with one, after
+        // (the record's rays keep their slots, and the barrier need not wait
+        // for this work).
+        V3f d = v3f(0.0f, 0.0f, 0.0f), phit = d, ns = d, wb = d;
+        bool trace_wb = false;
+        if (!ENV && act && h.x >= 0) vertex_dirs(D, rays, j, h, ts, want_rec, key, d, phit, ns, wb, trace_wb);
+        uint32_t cslot, rslot, bslot;
+        block_slots<!ENV>(S.cnt[nxt], want_cont, want_rec, trace_wb, (uint32_t)rf, cslot, rslot, bslot);
         do
         {
             if (!act) break;
@@ -583,12 +659,8 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 }
                 break;
             }
-            const float4 ra = rays[2 * j], rb = rays[2 * j + 1];
-            const V3f o = f3(ra), d = f3(rb);
-            const float t = float_of((uint32_t)h.y), u = float_of((uint32_t)h.z), v = float_of((uint32_t)h.w);
+            if (ENV) vertex_dirs(D, rays, j, h, ts, want_rec, key, d, phit, ns, wb, trace_wb);
             const V3f col = ld3(mt.rgb);
-            const V3f phit = o + d * t;
-            const V3f ns = shading_normal(ts, u, v);
             const bool no_direct = !sp.nee || k == 0 || !S.last_diff[p];
             V3f T = f3(S.T[p]);
             if (mt.type == MCRT_LGHT)
@@ -620,17 +692,16 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
             S.last_diff[p] = mt.type == MCRT_DIFF;
             if (want_rec)
             {
-                const int64_t m = rslot / (uint32_t)rs;
+                const int64_t m = rslot / (uint32_t)rf;
                 const int64_t r0i = a.P + (int64_t)rslot;
                 float4* R = S.rec[nxt] + rec_words(rs) * m;
                 float4* nr = S.rays[nxt];
                 const V3f zero = v3f(0.0f, 0.0f, 0.0f);
-                V3f wb = zero, wo = zero;
+                V3f wo = zero;
                 float pdf_b = 1.0f, factor2 = 0.0f;
                 int li_rec = -1;
                 if (D.nlights > 0)
                 {
-                    wb = lambert_dir(ns, rng_u01(key, 0), rng_u01(key, 1));
                     pdf_b = fmax_ref(dot(wb, ns), kEpsCos) / kPi;
                     // Scene::sampleLightArea (scene.cpp:298-343)
                     const int li = sample_light(D.lights, D.nlights, rng_u01(key, 2));
@@ -659,27 +730,20 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 }
                 R[0] = make_float4(T.x, T.y, T.z, pdf_b);
                 R[1] = make_float4(col.x, col.y, col.z, factor2);
-                if (!ENV) R[2] = make_float4(float_of(p), float_of((uint32_t)li_rec), 0.0f, 0.0f);
-                // The BSDF-sampled ray adds a term only when its closest hit is
-                // an emitter, and the traversal accepts a triangle only after its
-                // reference's box passes the slab test.  That test is monotone in
-                // the box (round-to-nearest keeps (b - o) * inv_dir ordered in b),
-                // so a ray failing it against the box around every emitter
-                // reference (with tmax = inf) reaches none, and its terms are a
-                // miss's: it is not traced.  Rays with an infinite inv_dir take
-                // the reference's NaN-prone sequential test and are always traced.
-                bool trace_wb = false;
-                if (D.emit_any)
-                {
-                    const RayPrep rw = prep_ray(phit.x, phit.y, phit.z, wb.x, wb.y, wb.z, kEpsRay);
-                    float lo;
-                    trace_wb = rw.exact || slab(rw, D.emit_lo[0], D.emit_hi[0], D.emit_lo[1], D.emit_hi[1],
-                                                D.emit_lo[2], D.emit_hi[2], __builtin_inff(), lo);
-                }
+                if (!ENV)
+                    R[2] = make_float4(float_of(p), float_of((uint32_t)li_rec), float_of(trace_wb ? bslot : ~0u), 0.0f);
                 // a zero direction fails every slab test (ray.cpp:77-79): the
                 // rays a vertex does not need cost one root visit
-                store_ray(nr, r0i, phit, trace_wb ? wb : zero, kEpsRay, __builtin_inff());
-                store_ray(nr, r0i + 1, phit, li_rec >= 0 ? wo : zero, kEpsRay, __builtin_inff());
+                if (ENV)
+                {
+                    store_ray(nr, r0i, phit, trace_wb ? wb : zero, kEpsRay, __builtin_inff());
+                    store_ray(nr, r0i + 1, phit, li_rec >= 0 ? wo : zero, kEpsRay, __builtin_inff());
+                }
+                else
+                {
+                    store_ray(nr, r0i, phit, li_rec >= 0 ? wo : zero, kEpsRay, __builtin_inff());
+                    if (trace_wb) store_ray(nr, a.P + rf * a.P + bslot, phit, wb, kEpsRay, __builtin_inff());
+                }
                 if (ENV)
                 {
                     V3f e1 = zero, e2 = zero, d1 = zero, d2 = zero;
@@ -690,7 +754,10 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                     store_ray(nr, r0i + 3, phit, d2, kEpsRay, __builtin_inff());
                 }
                 if (BATCH)
-                    for (int i = 0; i < rs; i++) S.rsid[nxt][r0i + i] = sid;
+                {
+                    for (int i = 0; i < rf; i++) S.rsid[nxt][r0i + i] = sid;
+                    if (!ENV && trace_wb) S.rsid[nxt][a.P + rf * a.P + bslot] = sid;
+                }
             }
             if (!want_cont) break;
             // Continuation (Material::sample, demo_pt.cpp:645-650)
@@ -840,9 +907,18 @@ void bounce_loop(const DevScene& Dt, PTState& S, const PTArgs& a, int max_depth,
         TraceBatch tb;
         tb.roots = a.roots;
         tb.rsid = S.rsid[cur];
-        launch_k_trace<false, false, true, BATCH>(Dt, trace_blocks(Dt, (1 + a.rs) * a.P), st, S.rays[cur], (int64_t)0,
-                                                  S.cnt[cur], a.P, S.hits, (uint8_t*)nullptr,
-                                                  (unsigned long long*)nullptr, S.cnt[cur] + kCntWq, tb);
+        // with an environment map every next-event ray sits at its record's
+        // slots (two segments); without, the BSDF-sampled rays follow in a
+        // third, compacted one
+        if (a.rs == 4)
+            launch_k_trace<false, false, 1, BATCH>(Dt, trace_blocks(Dt, (1 + a.rs) * a.P), st, S.rays[cur], (int64_t)0,
+                                                   S.cnt[cur], a.P, (int64_t)0, S.hits, (uint8_t*)nullptr,
+                                                   (unsigned long long*)nullptr, S.cnt[cur] + kCntWq, tb);
+        else
+            launch_k_trace<false, false, 2, BATCH>(Dt, trace_blocks(Dt, (1 + a.rs) * a.P), st, S.rays[cur], (int64_t)0,
+                                                   S.cnt[cur], a.P, a.P + rec_rays(a.rs) * a.P, S.hits,
+                                                   (uint8_t*)nullptr, (unsigned long long*)nullptr, S.cnt[cur] + kCntWq,
+                                                   tb);
         if (a.rs == 4)
             hipLaunchKernelGGL((k_resolve<ADJ, BATCH, true>), (grid_of<k_resolve<ADJ, BATCH, true>>(a.P, cus)),
                                dim3(kTPB), 0, st, a, S, cur);

@@ -18,11 +18,13 @@
 // at a time keeps the counters off the critical path (the path tracer's
 // launches hold up to 5 rays per path, most of them short).  Two ray
 // sources:
-//   * SEG = false: rays[0, n) with n known to the host;
-//   * SEG = true : two segments whose sizes live in device memory (the
-//     path tracer's compacted continuation rays [0, cnt[0]) and its
-//     next-event rays [seg1, seg1 + cnt[1])), so the bounce loop needs no
-//     host round trip and can be captured in a graph.
+//   * SEG = 0: rays[0, n) with n known to the host;
+//   * SEG = 1 or 2: segments whose sizes live in device memory (the path
+//     tracer's compacted continuation rays [0, cnt[0]) and its next-event
+//     rays at their records' slots [seg1, seg1 + cnt[1]); with SEG = 2 also
+//     its compacted BSDF-sampled next-event rays [seg2, seg2 +
+//     cnt[kSeg2Word])), so the bounce loop needs no host round trip and can
+//     be captured in a graph.
 // BATCH = true: the rays belong to several scenes (batched rendering); ray
 // i is traversed in the BVH of scene rsid[i].
 #pragma once
@@ -32,6 +34,9 @@ namespace mcrt
 {
 constexpr int kWave = 64;
 constexpr uint32_t kClaim = 128;
+// Word of a segment-mode count array holding the third segment's size (on
+// its own 128 B line: the path tracer takes it with its own atomic).
+constexpr int kSeg2Word = 32;
 
 // Rays of several scenes in one launch: ray i belongs to scene rsid[i]
 // (BATCH).  ORDER (queries): unless *skip is set at launch, the k-th ray
@@ -50,10 +55,10 @@ struct TraceBatch
 // registers into vector lanes, and the spill code in the refill block costs
 // more than the eighth wave gains).  Deeper stacks are held to fewer waves
 // by LDS.
-template <int D, bool ANY, bool COUNT, bool SEG, bool BATCH, bool ORDER>
+template <int D, bool ANY, bool COUNT, int SEG, bool BATCH, bool ORDER>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     D <= 8 ? (ANY ? kTraceAnyWavesPerSimd : kTraceWavesPerSimd) : 4, 8))) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
-                                                 const uint32_t* __restrict__ seg_cnt, int64_t seg1,
+                                                 const uint32_t* __restrict__ seg_cnt, int64_t seg1, int64_t seg2,
                                                  int4* __restrict__ hits, uint8_t* __restrict__ occ,
                                                  unsigned long long* __restrict__ counters, TraceBatch B,
                                                  uint32_t* __restrict__ wq)
@@ -72,11 +77,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     // 32-bit ray indices: a launch covers at most 2^31 rays (launch_trace
     // splits larger batches; path-tracer chunks are at most 5 * 2^28 rays),
     // and a slice counter ends below its slice + 2 * kClaim * (waves of the grid).
-    uint32_t n0 = (uint32_t)n, total = (uint32_t)n;
+    uint32_t n0 = (uint32_t)n, n01 = (uint32_t)n, total = (uint32_t)n;
     if (SEG)
     {
         n0 = seg_cnt[0];
-        total = n0 + seg_cnt[1];
+        n01 = n0 + seg_cnt[1];
+        total = SEG == 2 ? n01 + seg_cnt[kSeg2Word] : n01;
     }
     TraceCount cnt;
     uint32_t nrays = 0;
@@ -142,7 +148,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
             if (!live && rank < avail)
             {
                 const uint32_t k = claim + rank;
-                i = (SEG && k >= n0) ? (uint32_t)seg1 + (k - n0) : (ORDER && order != nullptr) ? order[k] : k;
+                if (SEG == 2)
+                    i = k < n0 ? k : k < n01 ? (uint32_t)seg1 + (k - n0) : (uint32_t)seg2 + (k - n01);
+                else if (SEG)
+                    i = k < n0 ? k : (uint32_t)seg1 + (k - n0);
+                else
+                    i = (ORDER && order != nullptr) ? order[k] : k;
                 if (BATCH) root = B.roots[B.rsid[i]];
                 const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
                 RayPrep rp = prep_ray(a.x, a.y, a.z, b.x, b.y, b.z, a.w);
@@ -189,28 +200,28 @@ inline int64_t trace_blocks(const DevScene& s, int64_t n)
 
 // Launches k_trace with the scene's LDS stack depth on `blocks` workgroups;
 // wq holds kWqWords words, the slice counters zero.
-template <bool ANY, bool COUNT, bool SEG, bool BATCH = false, bool ORDER = false>
+template <bool ANY, bool COUNT, int SEG, bool BATCH = false, bool ORDER = false>
 inline void launch_k_trace(const DevScene& S, int64_t blocks, hipStream_t st, const float4* rays, int64_t n,
-                           const uint32_t* seg_cnt, int64_t seg1, int4* hits, uint8_t* occ,
+                           const uint32_t* seg_cnt, int64_t seg1, int64_t seg2, int4* hits, uint8_t* occ,
                            unsigned long long* counters, uint32_t* wq, TraceBatch B = TraceBatch())
 {
     const dim3 g((unsigned)blocks), b(kWave);
     switch (S.lds_depth)
     {
     case 8:
-        hipLaunchKernelGGL((k_trace<8, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+        hipLaunchKernelGGL((k_trace<8, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, seg2, hits, occ,
                            counters, B, wq);
         break;
     case 12:
-        hipLaunchKernelGGL((k_trace<12, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+        hipLaunchKernelGGL((k_trace<12, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, seg2, hits, occ,
                            counters, B, wq);
         break;
     case 24:
-        hipLaunchKernelGGL((k_trace<24, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+        hipLaunchKernelGGL((k_trace<24, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, seg2, hits, occ,
                            counters, B, wq);
         break;
     default:
-        hipLaunchKernelGGL((k_trace<16, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, hits, occ,
+        hipLaunchKernelGGL((k_trace<16, ANY, COUNT, SEG, BATCH, ORDER>), g, b, 0, st, S, rays, n, seg_cnt, seg1, seg2, hits, occ,
                            counters, B, wq);
         break;
     }
