@@ -74,6 +74,42 @@ def test_render_bit_exact_two_separated_lights(ctx):
         _compare(f"two lights nee={nee}", g, o)
 
 
+def test_render_bit_exact_emissive_floor(ctx):
+    """The floor made an emitter as well: the box around the emitter
+    references spans the whole room, so nearly every BSDF-sampled next-event
+    ray is traced and the compacted segment of the extend launches holds
+    about one ray per record (the Cornell light alone leaves it nearly
+    empty).  Films equal the oracle's, with and without next-event
+    estimation, and in a batch with the unmodified scene."""
+    from mcrt_amd import api, scenes
+    import mcrt_amd
+    d = scenes.cornell_icosphere(2)
+    t = d.tris.reshape(-1, 3, 3)
+    floor = np.all(np.abs(t[:, :, 1] - t[:, :, 1].min()) < 1e-3, axis=1)
+    assert floor.sum() == 2
+    m = len(d.mat_type)
+    e = scenes.SceneData(d.tris, d.normals, np.where(floor, m, d.tri_mat), np.concatenate([d.mat_type, [api.LGHT]]),
+                         np.concatenate([d.mat_rgb, [(0.6, 0.9, 1.2)]]), np.concatenate([d.mat_gloss, [25.0]]))
+    films = {}
+    for nee in (True, False):
+        kw = dict(spp=3, max_depth=6, rr_start=3, rr_prob=0.8, seed=17, nee=nee)
+        print(f"[emissive floor nee={nee}]", flush=True)
+        g = gpu_render(ctx, e, 40, 32, **kw)
+        o = oracle_render(e, 40, 32, **kw)
+        assert g.mean() > 0.01
+        _compare(f"emissive floor nee={nee}", g, o)
+        films[nee] = g
+    print("[emissive floor batch]", flush=True)
+    kw = dict(spp=3, max_depth=6, rr_start=3, rr_prob=0.8, seed=17, nee=True)
+    sa = e.to_scene(ctx, width=40, height=32)
+    sb = d.to_scene(ctx, width=24, height=16)
+    fa = np.zeros((32, 40, 3), np.float32)
+    fb = np.zeros((16, 24, 3), np.float32)
+    mcrt_amd.render_batch([sa, sb], [kw, kw], [fa, fb])
+    _compare("emissive floor batched", fa, films[True])
+    _compare("unmodified scene batched", fb, oracle_render(d, 24, 16, **kw))
+
+
 def test_render_single_spp_no_filter(ctx):
     from mcrt_amd import scenes
     d = scenes.cornell_icosphere(1)
