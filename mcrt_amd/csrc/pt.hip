@@ -24,11 +24,10 @@
 // and the light-sampled shadow ray of the area lights, and, with an
 // environment map (rs = 4), the shadow rays of its two environment samples
 // (one drawn from the map's pdf, one from the cosine lobe; demo_pt.cpp:
-// 421-499, 564-589).  The record's rays sit at its slots, except that
-// without an environment map the BSDF-sampled ray is traced only where it
-// can reach an emitter and then goes to a compacted segment of its own (its
-// slot kept in the record).  A ray a scene does not need has a zero
-// direction, which every box rejects.
+// 421-499, 564-589).  The record's other rays sit at its slots; the
+// BSDF-sampled ray is traced only where it can reach an emitter and then
+// goes to a compacted segment of its own, its slot kept with the record.  A
+// ray a scene does not need has a zero direction, which every box rejects.
 //
 // Batched rendering (BATCH = true): the paths of many scenes -- each with
 // its own BVH, camera, film size, spp and depth -- share the chunks.  A
@@ -68,9 +67,11 @@ struct PTState
     float4* L;           // radiance (xyz)
     uint8_t* last_diff;  // previous vertex was diffuse
     float4* rays[2];     // [P continuation | rec_rays(rs) P next-event at the records' slots |
-                         //  P BSDF-sampled next-event, compacted (rs = 2)] x 2 float4 per ray
+                         //  P BSDF-sampled next-event, compacted] x 2 float4 per ray
     uint32_t* act[2];    // path id per continuation ray
     float4* rec[2];      // next-event records, rec_words(rs) float4 each
+    uint32_t* bsl[2];    // with an environment map: each record's BSDF-sampled ray slot or ~0
+                         // (its 64 B record has no spare word)
     uint32_t* cnt[2];    // {continuation rays, next-event rays at record slots}, the
                          // compacted BSDF-sampled rays at word kSeg2Word, then the
                          // traversal's work counters from word kCntWq on (kSideWords
@@ -93,9 +94,10 @@ struct PTState
 // are read back from the rays themselves (the resolve step needs them only
 // where a ray hit an emitter).
 __host__ __device__ constexpr int rec_words(int rs) { return rs == 4 ? 4 : 3; }
-// Rays a record keeps at its own slots: the light sample's shadow ray (the
-// BSDF-sampled ray is compacted), or with an environment map all four.
-__host__ __device__ constexpr int rec_rays(int rs) { return rs == 4 ? 4 : 1; }
+// Rays a record keeps at its own slots: the light sample's shadow ray and,
+// with an environment map, its two environment shadow rays (the
+// BSDF-sampled ray is compacted).
+__host__ __device__ constexpr int rec_rays(int rs) { return rs - 1; }
 
 // One scene of a render: its device data, camera and parameters.
 struct ScenePT
@@ -138,20 +140,18 @@ __device__ __forceinline__ void store_ray(float4* rays, int64_t i, V3f o, V3f d,
 }
 __device__ __forceinline__ V3f f3(float4 v) { return V3f{v.x, v.y, v.z}; }
 
-// Slots of this thread in the workgroup's two compactions of one
-// iteration: continuation rays (counter c[0], one slot each) and
-// next-event records of `per` rays each (counter c[1], in rays; sb is the
-// record's first ray slot).  One 64-bit atomic on the two adjacent
-// counters per workgroup and iteration: device-scope atomics on one
-// address serialise (about 12 ns each on MI355X), so one per wavefront and
-// counter -- 2^19 for 2^24 paths -- would take over 5 ms by themselves.
-// Consecutive threads get consecutive slots, so the stores that follow
-// coalesce.  Every thread of the workgroup calls it (it synchronises the
-// workgroup).  Slot order never affects a result: every path's records
-// are its own.
-// With E, a third compaction (counter c[kSeg2Word], on its own cache line,
-// one slot each, slot se) by a second atomic issued beside the first.
-template <bool E>
+// Slots of this thread in the workgroup's three compactions of one
+// iteration: continuation rays (counter c[0], one slot each), next-event
+// records of `per` rays each (counter c[1], in rays; sb is the record's
+// first ray slot) and traced BSDF-sampled next-event rays (counter
+// c[kSeg2Word], one slot each).  One 64-bit atomic on the two adjacent
+// counters and one on the third, on a cache line of its own, per workgroup
+// and iteration: device-scope atomics on one address serialise (about 12 ns
+// each on MI355X), so one per wavefront and counter -- 2^19 for 2^24 paths
+// -- would take over 5 ms by themselves.  Consecutive threads get
+// consecutive slots, so the stores that follow coalesce.  Every thread of
+// the workgroup calls it (it synchronises the workgroup).  Slot order never
+// affects a result: every path's records are its own.
 __device__ __forceinline__ void block_slots(uint32_t* c, bool a, bool b, bool e, uint32_t per, uint32_t& sa,
                                             uint32_t& sb, uint32_t& se)
 {
@@ -159,19 +159,19 @@ __device__ __forceinline__ void block_slots(uint32_t* c, bool a, bool b, bool e,
     __shared__ uint32_t s_n[3][W];
     __shared__ unsigned long long s_base;
     __shared__ uint32_t s_base_e;
-    const uint64_t ma = __ballot(a), mb = __ballot(b), me = E ? __ballot(e) : 0ull;
+    const uint64_t ma = __ballot(a), mb = __ballot(b), me = __ballot(e);
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     if (lane == 0)
     {
         s_n[0][w] = (uint32_t)__popcll(ma);
         s_n[1][w] = (uint32_t)__popcll(mb);
-        if (E) s_n[2][w] = (uint32_t)__popcll(me);
+        s_n[2][w] = (uint32_t)__popcll(me);
     }
     __syncthreads();
     uint32_t pa = 0, pb = 0, pe = 0, ta = 0, tb = 0, te = 0;
     for (uint32_t q = 0; q < W; q++)
     {
-        const uint32_t na = s_n[0][q], nb = s_n[1][q], ne = E ? s_n[2][q] : 0u;
+        const uint32_t na = s_n[0][q], nb = s_n[1][q], ne = s_n[2][q];
         pa += q < w ? na : 0u;
         pb += q < w ? nb : 0u;
         pe += q < w ? ne : 0u;
@@ -185,7 +185,7 @@ __device__ __forceinline__ void block_slots(uint32_t* c, bool a, bool b, bool e,
     {
         if ((ta | tb) != 0)
             s_base = atomicAdd((unsigned long long*)c, (unsigned long long)ta | ((unsigned long long)(tb * per) << 32));
-        if (E && te != 0) s_base_e = atomicAdd(c + kSeg2Word, te);
+        if (te != 0) s_base_e = atomicAdd(c + kSeg2Word, te);
     }
     __syncthreads();
     // s_n and the bases are rewritten only after the next call's first
@@ -194,7 +194,7 @@ __device__ __forceinline__ void block_slots(uint32_t* c, bool a, bool b, bool e,
     const uint64_t below = (1ull << lane) - 1ull;
     sa = (uint32_t)base + pa + (uint32_t)__popcll(ma & below);
     sb = (uint32_t)(base >> 32) + (pb + (uint32_t)__popcll(mb & below)) * per;
-    se = E ? s_base_e + pe + (uint32_t)__popcll(me & below) : 0u;
+    se = s_base_e + pe + (uint32_t)__popcll(me & below);
 }
 __device__ __forceinline__ float4 f4(V3f v) { return make_float4(v.x, v.y, v.z, 0.0f); }
 
@@ -467,23 +467,22 @@ __global__ void k_resolve(PTArgs a, PTState S, int cur)
             if (ENV)
             {
                 const V3f z = v3f(0.0f, 0.0f, 0.0f);
-                const V3f e1 = S.hits[h0 + 2].x < 0 ? f3(r2) : z;
-                const V3f e2 = S.hits[h0 + 3].x < 0 ? f3(r3) : z;
+                const V3f e1 = S.hits[h0 + 1].x < 0 ? f3(r2) : z;
+                const V3f e2 = S.hits[h0 + 2].x < 0 ? f3(r3) : z;
                 local = local + (e1 + e2);
             }
             if (D.nlights > 0)
             {
                 const int li = (int)bits_of(ENV ? r3.w : r2.y);
                 const int4 miss = make_int4(-1, 0, 0, 0);
-                // the BSDF-sampled ray: at the record's first slot, or (without
-                // an environment map) in the compacted segment after the
-                // records' slots where it was traced, else a miss
-                const int bs = ENV ? 0 : (int)bits_of(r2.z);
-                const int64_t hb = ENV ? h0 : P + rf * P + bs, hl = ENV ? h0 + 1 : h0;
+                // the BSDF-sampled ray in the compacted segment after the
+                // records' slots where it was traced, else a miss; the light
+                // sample's shadow ray at the record's first slot
+                const int bs = ENV ? (int)S.bsl[cur][m] : (int)bits_of(r2.z);
+                const int64_t hb = P + rf * P + bs;
                 V3f c1, c2;
-                local = area_terms(D, (ENV || bs >= 0) ? S.hits[hb] : miss, li >= 0 ? S.hits[hl] : miss, col,
-                                   rays + 2 * hb, rays + 2 * hl, r0.w, r1.w, li, local, em1, em2, hot1, hot2, c1,
-                                   c2);
+                local = area_terms(D, bs >= 0 ? S.hits[hb] : miss, li >= 0 ? S.hits[h0] : miss, col, rays + 2 * hb,
+                                   rays + 2 * h0, r0.w, r1.w, li, local, em1, em2, hot1, hot2, c1, c2);
                 if (ADJ)
                 {
                     const V3f dl = f3(S.delta[path]);
@@ -636,16 +635,14 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
             }
         }
         // The hit point, the shading normal and the BSDF-sampled next-event
-        // direction: without an environment map they come before the
-        // workgroup takes its slots, since whether that ray is traced decides
-        // whether it takes a slot of the compacted segment; with one, after
-        // (the record's rays keep their slots, and the barrier need not wait
-        // for this work).
+        // direction come before the workgroup takes its slots: whether that
+        // ray is traced decides whether it takes a slot of the compacted
+        // segment.
         V3f d = v3f(0.0f, 0.0f, 0.0f), phit = d, ns = d, wb = d;
         bool trace_wb = false;
-        if (!ENV && act && h.x >= 0) vertex_dirs(D, rays, j, h, ts, want_rec, key, d, phit, ns, wb, trace_wb);
+        if (act && h.x >= 0) vertex_dirs(D, rays, j, h, ts, want_rec, key, d, phit, ns, wb, trace_wb);
         uint32_t cslot, rslot, bslot;
-        block_slots<!ENV>(S.cnt[nxt], want_cont, want_rec, trace_wb, (uint32_t)rf, cslot, rslot, bslot);
+        block_slots(S.cnt[nxt], want_cont, want_rec, trace_wb, (uint32_t)rf, cslot, rslot, bslot);
         do
         {
             if (!act) break;
@@ -659,7 +656,6 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 }
                 break;
             }
-            if (ENV) vertex_dirs(D, rays, j, h, ts, want_rec, key, d, phit, ns, wb, trace_wb);
             const V3f col = ld3(mt.rgb);
             const bool no_direct = !sp.nee || k == 0 || !S.last_diff[p];
             V3f T = f3(S.T[p]);
@@ -730,33 +726,28 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 }
                 R[0] = make_float4(T.x, T.y, T.z, pdf_b);
                 R[1] = make_float4(col.x, col.y, col.z, factor2);
-                if (!ENV)
-                    R[2] = make_float4(float_of(p), float_of((uint32_t)li_rec), float_of(trace_wb ? bslot : ~0u), 0.0f);
+                const uint32_t bs = trace_wb ? bslot : ~0u;
+                if (ENV)
+                    S.bsl[nxt][m] = bs;
+                else
+                    R[2] = make_float4(float_of(p), float_of((uint32_t)li_rec), float_of(bs), 0.0f);
                 // a zero direction fails every slab test (ray.cpp:77-79): the
                 // rays a vertex does not need cost one root visit
-                if (ENV)
-                {
-                    store_ray(nr, r0i, phit, trace_wb ? wb : zero, kEpsRay, __builtin_inff());
-                    store_ray(nr, r0i + 1, phit, li_rec >= 0 ? wo : zero, kEpsRay, __builtin_inff());
-                }
-                else
-                {
-                    store_ray(nr, r0i, phit, li_rec >= 0 ? wo : zero, kEpsRay, __builtin_inff());
-                    if (trace_wb) store_ray(nr, a.P + rf * a.P + bslot, phit, wb, kEpsRay, __builtin_inff());
-                }
+                store_ray(nr, r0i, phit, li_rec >= 0 ? wo : zero, kEpsRay, __builtin_inff());
+                if (trace_wb) store_ray(nr, a.P + rf * a.P + bslot, phit, wb, kEpsRay, __builtin_inff());
                 if (ENV)
                 {
                     V3f e1 = zero, e2 = zero, d1 = zero, d2 = zero;
                     if (D.env_w > 0) env_terms(D, key, ns, col, e1, e2, d1, d2);
                     R[2] = make_float4(e1.x, e1.y, e1.z, float_of(p));
                     R[3] = make_float4(e2.x, e2.y, e2.z, float_of((uint32_t)li_rec));
-                    store_ray(nr, r0i + 2, phit, d1, kEpsRay, __builtin_inff());
-                    store_ray(nr, r0i + 3, phit, d2, kEpsRay, __builtin_inff());
+                    store_ray(nr, r0i + 1, phit, d1, kEpsRay, __builtin_inff());
+                    store_ray(nr, r0i + 2, phit, d2, kEpsRay, __builtin_inff());
                 }
                 if (BATCH)
                 {
                     for (int i = 0; i < rf; i++) S.rsid[nxt][r0i + i] = sid;
-                    if (!ENV && trace_wb) S.rsid[nxt][a.P + rf * a.P + bslot] = sid;
+                    if (trace_wb) S.rsid[nxt][a.P + rf * a.P + bslot] = sid;
                 }
             }
             if (!want_cont) break;
@@ -907,18 +898,9 @@ void bounce_loop(const DevScene& Dt, PTState& S, const PTArgs& a, int max_depth,
         TraceBatch tb;
         tb.roots = a.roots;
         tb.rsid = S.rsid[cur];
-        // with an environment map every next-event ray sits at its record's
-        // slots (two segments); without, the BSDF-sampled rays follow in a
-        // third, compacted one
-        if (a.rs == 4)
-            launch_k_trace<false, false, 1, BATCH>(Dt, trace_blocks(Dt, (1 + a.rs) * a.P), st, S.rays[cur], (int64_t)0,
-                                                   S.cnt[cur], a.P, (int64_t)0, S.hits, (uint8_t*)nullptr,
-                                                   (unsigned long long*)nullptr, S.cnt[cur] + kCntWq, tb);
-        else
-            launch_k_trace<false, false, 2, BATCH>(Dt, trace_blocks(Dt, (1 + a.rs) * a.P), st, S.rays[cur], (int64_t)0,
-                                                   S.cnt[cur], a.P, a.P + rec_rays(a.rs) * a.P, S.hits,
-                                                   (uint8_t*)nullptr, (unsigned long long*)nullptr, S.cnt[cur] + kCntWq,
-                                                   tb);
+        launch_k_trace<false, false, true, BATCH>(Dt, trace_blocks(Dt, (1 + a.rs) * a.P), st, S.rays[cur], (int64_t)0,
+                                                  S.cnt[cur], a.P, a.P + rec_rays(a.rs) * a.P, S.hits, (uint8_t*)nullptr,
+                                                  (unsigned long long*)nullptr, S.cnt[cur] + kCntWq, tb);
         if (a.rs == 4)
             hipLaunchKernelGGL((k_resolve<ADJ, BATCH, true>), (grid_of<k_resolve<ADJ, BATCH, true>>(a.P, cus)),
                                dim3(kTPB), 0, st, a, S, cur);
@@ -951,6 +933,7 @@ PTState state_of(StreamRes* r, bool adjoint)
         S.rays[i] = w.rays[i].as<float4>();
         S.act[i] = w.act[i].as<uint32_t>();
         S.rec[i] = w.rec[i].as<float4>();
+        S.bsl[i] = w.bsl[i].as<uint32_t>();
         S.cnt[i] = w.cnt.as<uint32_t>() + kSideWords * i;
         S.rsid[i] = w.rsid[i].as<uint16_t>();
     }
@@ -998,6 +981,7 @@ int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch, int rs)
         if ((rc = w.rays[i].ensure(nr * 32))) return rc;
         if ((rc = w.act[i].ensure(n * 4))) return rc;
         if ((rc = w.rec[i].ensure(n * 16 * rec_words(rs)))) return rc;
+        if (rs == 4 && (rc = w.bsl[i].ensure(n * 4))) return rc;
     }
     if ((rc = w.cnt.ensure(2 * kSideWords * sizeof(uint32_t)))) return rc;
     if ((rc = w.hits.ensure(nr * 16))) return rc;

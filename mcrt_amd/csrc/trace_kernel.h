@@ -18,13 +18,12 @@
 // at a time keeps the counters off the critical path (the path tracer's
 // launches hold up to 5 rays per path, most of them short).  Two ray
 // sources:
-//   * SEG = 0: rays[0, n) with n known to the host;
-//   * SEG = 1 or 2: segments whose sizes live in device memory (the path
-//     tracer's compacted continuation rays [0, cnt[0]) and its next-event
-//     rays at their records' slots [seg1, seg1 + cnt[1]); with SEG = 2 also
-//     its compacted BSDF-sampled next-event rays [seg2, seg2 +
-//     cnt[kSeg2Word])), so the bounce loop needs no host round trip and can
-//     be captured in a graph.
+//   * SEG = false: rays[0, n) with n known to the host;
+//   * SEG = true : three segments whose sizes live in device memory (the
+//     path tracer's compacted continuation rays [0, cnt[0]), its next-event
+//     rays at their records' slots [seg1, seg1 + cnt[1]) and its compacted
+//     BSDF-sampled next-event rays [seg2, seg2 + cnt[kSeg2Word])), so the
+//     bounce loop needs no host round trip and can be captured in a graph.
 // BATCH = true: the rays belong to several scenes (batched rendering); ray
 // i is traversed in the BVH of scene rsid[i].
 #pragma once
@@ -55,7 +54,7 @@ struct TraceBatch
 // registers into vector lanes, and the spill code in the refill block costs
 // more than the eighth wave gains).  Deeper stacks are held to fewer waves
 // by LDS.
-template <int D, bool ANY, bool COUNT, int SEG, bool BATCH, bool ORDER>
+template <int D, bool ANY, bool COUNT, bool SEG, bool BATCH, bool ORDER>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     D <= 8 ? (ANY ? kTraceAnyWavesPerSimd : kTraceWavesPerSimd) : 4, 8))) void k_trace(DevScene S, const float4* __restrict__ rays, int64_t n,
                                                  const uint32_t* __restrict__ seg_cnt, int64_t seg1, int64_t seg2,
@@ -82,7 +81,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
     {
         n0 = seg_cnt[0];
         n01 = n0 + seg_cnt[1];
-        total = SEG == 2 ? n01 + seg_cnt[kSeg2Word] : n01;
+        total = n01 + seg_cnt[kSeg2Word];
     }
     TraceCount cnt;
     uint32_t nrays = 0;
@@ -148,10 +147,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(
             if (!live && rank < avail)
             {
                 const uint32_t k = claim + rank;
-                if (SEG == 2)
+                if (SEG)
                     i = k < n0 ? k : k < n01 ? (uint32_t)seg1 + (k - n0) : (uint32_t)seg2 + (k - n01);
-                else if (SEG)
-                    i = k < n0 ? k : (uint32_t)seg1 + (k - n0);
                 else
                     i = (ORDER && order != nullptr) ? order[k] : k;
                 if (BATCH) root = B.roots[B.rsid[i]];
@@ -200,7 +197,7 @@ inline int64_t trace_blocks(const DevScene& s, int64_t n)
 
 // Launches k_trace with the scene's LDS stack depth on `blocks` workgroups;
 // wq holds kWqWords words, the slice counters zero.
-template <bool ANY, bool COUNT, int SEG, bool BATCH = false, bool ORDER = false>
+template <bool ANY, bool COUNT, bool SEG, bool BATCH = false, bool ORDER = false>
 inline void launch_k_trace(const DevScene& S, int64_t blocks, hipStream_t st, const float4* rays, int64_t n,
                            const uint32_t* seg_cnt, int64_t seg1, int64_t seg2, int4* hits, uint8_t* occ,
                            unsigned long long* counters, uint32_t* wq, TraceBatch B = TraceBatch())

@@ -180,11 +180,11 @@ hipError_t launch(const DevScene& s, const float* rays, int64_t n, void* hits, u
         TraceBatch tb;
         tb.order = s.order;
         tb.skip = s.cell_work + kCellSkip;
-        launch_k_trace<ANY, COUNT, 0, false, true>(s, trace_blocks(s, n), stream, r4, n, nullptr, (int64_t)0, (int64_t)0, h4,
+        launch_k_trace<ANY, COUNT, false, false, true>(s, trace_blocks(s, n), stream, r4, n, nullptr, (int64_t)0, (int64_t)0, h4,
                                                        occ, c, s.wq, tb);
     }
     else
-        launch_k_trace<ANY, COUNT, 0>(s, trace_blocks(s, n), stream, r4, n, nullptr, (int64_t)0, (int64_t)0, h4, occ, c,
+        launch_k_trace<ANY, COUNT, false>(s, trace_blocks(s, n), stream, r4, n, nullptr, (int64_t)0, (int64_t)0, h4, occ, c,
                                           s.wq);
     return hipGetLastError();
 }
