@@ -189,7 +189,10 @@ __device__ __forceinline__ void block_slots(uint32_t* c, bool a, bool b, bool e,
     }
     __syncthreads();
     // s_n and the bases are rewritten only after the next call's first
-    // barrier, which every thread reaches after these reads
+    // barrier, which every thread reaches after these reads.  Every lane
+    // reads the bases, but a slot means something only to a lane that asked
+    // for one, and then its workgroup's total is non-zero and the base was
+    // written above.
     const unsigned long long base = s_base;
     const uint64_t below = (1ull << lane) - 1ull;
     sa = (uint32_t)base + pa + (uint32_t)__popcll(ma & below);
