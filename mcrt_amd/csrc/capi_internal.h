@@ -65,7 +65,7 @@ namespace mcrt
 // Path-tracer workspace (SoA path state), grown on demand.
 struct PTWork
 {
-    DevBuf T, L, flags, rays[2], act[2], rec[2], bsl[2], cnt, hits, samples, film;
+    DevBuf T, L, flags, rays[2], act[2], rec[2], bsl[2], cnt, hits, film;
     DevBuf Ltot, delta;                    // adjoint
     DevBuf dfilm, galb, gem;               // adjoint staging for host pointers
     DevBuf sid, pkey, skey, rsid[2];       // batched rendering

@@ -15,8 +15,8 @@
 //     shade (k <= max)   emitter hit or escape to the environment,
 //                        next-event rays at diffuse vertices, BSDF
 //                        continuation, Russian roulette
-//   finish               NaN -> 0, clamp >= 0 per sample
-//   accumulate           per-pixel sum in sample order x (1/spp)
+//   accumulate           per-pixel sum in sample order of each path's
+//                        radiance (NaN -> 0, clamp >= 0) x (1/spp)
 // Every kernel is grid-strided over a fixed grid and reads its work count
 // from device memory, so the whole chunk is launch-only (graph-capturable).
 //
@@ -77,7 +77,6 @@ struct PTState
                          // traversal's work counters from word kCntWq on (kSideWords
                          // words per side)
     int4* hits;          // (1 + rs) P
-    float4* samples;     // P clamped sample radiance
     float4* Ltot;        // adjoint: radiance of pass 1
     float4* delta;       // adjoint: dLoss/dL per path
     float* g_alb;        // adjoint: ntri x 3
@@ -784,17 +783,6 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
     if (ADJ) hot_flush(a.one.D, acc, S.g_alb, S.g_em);
 }
 
-// demo_pt.cpp:653 (NaN -> 0) and :703 (clamp >= 0 per channel).
-__global__ void k_finish(PTState S, int64_t P)
-{
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (int64_t)gridDim.x * blockDim.x)
-    {
-        float4 L = S.L[p];
-        if (!(L.x == L.x && L.y == L.y && L.z == L.z)) L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        S.samples[p] = make_float4(fmax_ref(0.0f, L.x), fmax_ref(0.0f, L.y), fmax_ref(0.0f, L.z), 0.0f);
-    }
-}
-
 // Adjoint seed: keep pass 1's radiance and form dLoss/dL of each path
 // through the clamp (slope 1 where L >= 0), the NaN rule and the 1/spp
 // scale of the film.
@@ -812,9 +800,10 @@ __global__ void k_delta(PTState S, PTArgs a, const float* dfilm, float scale)
     }
 }
 
-// FrameBuffer::accumulatePixel over the samples in order, then
-// scale((float)(1.0/spp)) (demo_pt.cpp:700-710, film.cpp:204-260), for the
-// npix pixels of the chunk.
+// FrameBuffer::accumulatePixel over the samples in order, each path's
+// radiance taken with demo_pt.cpp:653 (NaN -> 0) and :703 (clamp >= 0 per
+// channel), then scale((float)(1.0/spp)) (demo_pt.cpp:700-710,
+// film.cpp:204-260), for the npix pixels of the chunk.
 template <bool BATCH>
 __global__ void k_accumulate(PTState S, PTArgs a, float* film, int64_t npix)
 {
@@ -844,10 +833,11 @@ __global__ void k_accumulate(PTState S, PTArgs a, float* film, int64_t npix)
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
         for (int s = 0; s < spp; s++)
         {
-            const float4 v = S.samples[p0 + s];
-            gx = gx + v.x;
-            gy = gy + v.y;
-            gz = gz + v.z;
+            float4 L = S.L[p0 + s];
+            if (!(L.x == L.x && L.y == L.y && L.z == L.z)) L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            gx = gx + fmax_ref(0.0f, L.x);
+            gy = gy + fmax_ref(0.0f, L.y);
+            gz = gz + fmax_ref(0.0f, L.z);
         }
         float* f = out + 3 * pixel;
         f[0] = gx * scale;
@@ -941,7 +931,6 @@ PTState state_of(StreamRes* r, bool adjoint)
         S.rsid[i] = w.rsid[i].as<uint16_t>();
     }
     S.hits = w.hits.as<int4>();
-    S.samples = w.samples.as<float4>();
     S.Ltot = adjoint ? w.Ltot.as<float4>() : nullptr;
     S.delta = adjoint ? w.delta.as<float4>() : nullptr;
     S.g_alb = S.g_em = nullptr;
@@ -988,7 +977,6 @@ int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch, int rs)
     }
     if ((rc = w.cnt.ensure(2 * kSideWords * sizeof(uint32_t)))) return rc;
     if ((rc = w.hits.ensure(nr * 16))) return rc;
-    if ((rc = w.samples.ensure(n * 16))) return rc;
     if (adjoint)
     {
         if ((rc = w.Ltot.ensure(n * 16))) return rc;
@@ -1050,7 +1038,6 @@ int render_scene(mcrt_scene* s, StreamRes* r, const mcrt_render_params* prm, flo
         }
         if (film)
         {
-            hipLaunchKernelGGL(k_finish, (grid_of<k_finish>(a.P, cus)), dim3(kTPB), 0, st, S, a.P);
             hipLaunchKernelGGL(k_accumulate<false>, (grid_of<k_accumulate<false>>(np, cus)), dim3(kTPB), 0, st, S, a,
                                film, np);
         }
@@ -1161,7 +1148,6 @@ int batch_enqueue(mcrt_batch* b, hipStream_t st)
         a.nseg = (int32_t)c.nseg;
         a.seg = b->d_seg.as<Segment>() + c.seg0;
         bounce_loop<false, true>(b->trace_view, S, a, b->max_depth, cus, st);
-        hipLaunchKernelGGL(k_finish, (grid_of<k_finish>(a.P, cus)), dim3(kTPB), 0, st, S, a.P);
         hipLaunchKernelGGL(k_accumulate<true>, (grid_of<k_accumulate<true>>(c.npix, cus)), dim3(kTPB), 0, st, S, a,
                            (float*)nullptr, c.npix);
         HIPCHK(hipGetLastError());
