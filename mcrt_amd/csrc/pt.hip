@@ -359,11 +359,11 @@ __global__ void k_generate(PTState S, PTArgs a)
         V3f sv = (ld3(cam.fy) * (my - 0.5f)) * 1.0f;
         V3f sw = ld3(cam.fwd) * cam.dist;
         V3f d = nrm((su + sv) + sw);
+        // camera ray p is path p, its throughput is one and no earlier
+        // vertex was diffuse: the first k_shade takes these as constants,
+        // so only the ray and the radiance are stored
         store_ray(S.rays[0], p, ld3(cam.o), d, kEpsRay, __builtin_inff());
-        S.act[0][p] = (uint32_t)p;
-        S.T[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         S.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        S.last_diff[p] = 0;
     }
 }
 
@@ -607,7 +607,7 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
         uint16_t sid = 0;
         if (act)
         {
-            p = S.act[cur][j];
+            p = k == 0 ? (uint32_t)j : S.act[cur][j];
             h = S.hits[j];
             if (BATCH) sid = S.sid[p];
         }
@@ -653,14 +653,15 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                 // escape: the passive environment term (demo_pt.cpp:528-540)
                 if (ENV && D.env_w > 0 && (!sp.nee || k == 0 || !S.last_diff[p]))
                 {
-                    const V3f Lv = f3(S.L[p]) + f3(S.T[p]) * env_rad(D, env_pixel(D, f3(rays[2 * j + 1])));
+                    const V3f T = k == 0 ? v3f(1.0f, 1.0f, 1.0f) : f3(S.T[p]);
+                    const V3f Lv = f3(S.L[p]) + T * env_rad(D, env_pixel(D, f3(rays[2 * j + 1])));
                     S.L[p] = f4(Lv);
                 }
                 break;
             }
             const V3f col = ld3(mt.rgb);
             const bool no_direct = !sp.nee || k == 0 || !S.last_diff[p];
-            V3f T = f3(S.T[p]);
+            V3f T = k == 0 ? v3f(1.0f, 1.0f, 1.0f) : f3(S.T[p]);
             if (mt.type == MCRT_LGHT)
             {
                 if (no_direct && dot(d, ns) < 0)
@@ -831,14 +832,26 @@ __global__ void k_accumulate(PTState S, PTArgs a, float* film, int64_t npix)
             out = film;
         }
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-        for (int s = 0; s < spp; s++)
-        {
-            float4 L = S.L[p0 + s];
+        auto add = [&](float4 L) {
             if (!(L.x == L.x && L.y == L.y && L.z == L.z)) L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             gx = gx + fmax_ref(0.0f, L.x);
             gy = gy + fmax_ref(0.0f, L.y);
             gz = gz + fmax_ref(0.0f, L.z);
+        };
+        // A lane's samples are consecutive; loading eight (128 B) before
+        // summing them lets one fetch of each line serve all eight, where
+        // one load per step saw the line evicted between steps.  The sum
+        // keeps the sample order.
+        int s = 0;
+        for (; s + 8 <= spp; s += 8)
+        {
+            float4 v[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = S.L[p0 + s + i];
+#pragma unroll
+            for (int i = 0; i < 8; i++) add(v[i]);
         }
+        for (; s < spp; s++) add(S.L[p0 + s]);
         float* f = out + 3 * pixel;
         f[0] = gx * scale;
         f[1] = gy * scale;
