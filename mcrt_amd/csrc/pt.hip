@@ -492,8 +492,16 @@ __global__ void k_resolve(PTArgs a, PTState S, int cur)
                     g2 = (dl * T) * c2;
                 }
             }
-            V3f Lv = f3(S.L[path]) + T * local;
-            S.L[path] = f4(Lv);
+            // A path's radiance starts at +0 and only ever has terms added,
+            // so it is never -0 and adding a zero term leaves its bits as
+            // they are: records whose term is zero (shadowed light sample,
+            // BSDF ray missing every emitter) skip the read and the write.
+            const V3f x = T * local;
+            if (!(x.x == 0.0f && x.y == 0.0f && x.z == 0.0f))
+            {
+                const V3f Lv = f3(S.L[path]) + x;
+                S.L[path] = f4(Lv);
+            }
         }
         if (ADJ)
         {
