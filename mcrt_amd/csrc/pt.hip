@@ -651,6 +651,16 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
         V3f d = v3f(0.0f, 0.0f, 0.0f), phit = d, ns = d, wb = d;
         bool trace_wb = false;
         if (act && h.x >= 0) vertex_dirs(D, rays, j, h, ts, want_rec, key, d, phit, ns, wb, trace_wb);
+        // The path's throughput and whether its previous vertex was diffuse
+        // are loaded before the slots too: their latency overlaps the slot
+        // atomic.  The first vertex has unit throughput and no predecessor.
+        V3f T = v3f(1.0f, 1.0f, 1.0f);
+        bool no_direct = true;
+        if (act && k != 0 && (h.x >= 0 || (ENV && D.env_w > 0)))
+        {
+            T = f3(S.T[p]);
+            no_direct = !sp.nee || !S.last_diff[p];
+        }
         uint32_t cslot, rslot, bslot;
         block_slots(S.cnt[nxt], want_cont, want_rec, trace_wb, (uint32_t)rf, cslot, rslot, bslot);
         do
@@ -659,17 +669,14 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
             if (h.x < 0)
             {
                 // escape: the passive environment term (demo_pt.cpp:528-540)
-                if (ENV && D.env_w > 0 && (!sp.nee || k == 0 || !S.last_diff[p]))
+                if (ENV && D.env_w > 0 && no_direct)
                 {
-                    const V3f T = k == 0 ? v3f(1.0f, 1.0f, 1.0f) : f3(S.T[p]);
                     const V3f Lv = f3(S.L[p]) + T * env_rad(D, env_pixel(D, f3(rays[2 * j + 1])));
                     S.L[p] = f4(Lv);
                 }
                 break;
             }
             const V3f col = ld3(mt.rgb);
-            const bool no_direct = !sp.nee || k == 0 || !S.last_diff[p];
-            V3f T = k == 0 ? v3f(1.0f, 1.0f, 1.0f) : f3(S.T[p]);
             if (mt.type == MCRT_LGHT)
             {
                 if (no_direct && dot(d, ns) < 0)
