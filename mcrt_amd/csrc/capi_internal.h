@@ -65,7 +65,7 @@ namespace mcrt
 // Path-tracer workspace (SoA path state), grown on demand.
 struct PTWork
 {
-    DevBuf T, L, flags, rays[2], act[2], rec[2], bsl[2], cnt, hits, film;
+    DevBuf T, L, rays[2], act[2], rec[2], bsl[2], cnt, hits, film;
     DevBuf Ltot, delta;                    // adjoint
     DevBuf dfilm, galb, gem;               // adjoint staging for host pointers
     DevBuf sid, pkey, skey, rsid[2];       // batched rendering
@@ -146,8 +146,8 @@ struct mcrt_scene
     int32_t trace_lds_depth = 8;
     int32_t pt_trace_waves_per_cu = 0;
     int32_t pt_trace_lds_depth = 8;
-    // 2^26 paths per chunk (a whole 1024^2 x 64-spp frame; about 28.5 GB
-    // of path state, 42.4 GB with an environment map): fewer, larger launches
+    // 2^26 paths per chunk (a whole 1024^2 x 64-spp frame; about 28.4 GB
+    // of path state, 42.3 GB with an environment map): fewer, larger launches
     // whose ragged ends weigh less (the frame 109 ms, against 121 ms in 2^24
     // chunks and 140 ms in 2^23)
     int64_t pt_chunk_paths = int64_t(1) << 26;

@@ -63,9 +63,8 @@ constexpr int kTPB = 256;
 
 struct PTState
 {
-    float4* T;           // throughput (xyz)
+    float4* T;           // throughput (xyz), w = 1 where the vertex it left was diffuse
     float4* L;           // radiance (xyz)
-    uint8_t* last_diff;  // previous vertex was diffuse
     float4* rays[2];     // [P continuation | rec_rays(rs) P next-event at the records' slots |
                          //  P BSDF-sampled next-event, compacted] x 2 float4 per ray
     uint32_t* act[2];    // path id per continuation ray
@@ -658,8 +657,9 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
         bool no_direct = true;
         if (act && k != 0 && (h.x >= 0 || (ENV && D.env_w > 0)))
         {
-            T = f3(S.T[p]);
-            no_direct = !sp.nee || !S.last_diff[p];
+            const float4 t4 = S.T[p];
+            T = f3(t4);
+            no_direct = !sp.nee || t4.w == 0.0f;
         }
         uint32_t cslot, rslot, bslot;
         block_slots(S.cnt[nxt], want_cont, want_rec, trace_wb, (uint32_t)rf, cslot, rslot, bslot);
@@ -703,7 +703,6 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
                             col.y > 0.0f ? dl.y * rest.y / col.y : 0.0f,
                             col.z > 0.0f ? dl.z * rest.z / col.z : 0.0f);
             }
-            S.last_diff[p] = mt.type == MCRT_DIFF;
             if (want_rec)
             {
                 const int64_t m = rslot / (uint32_t)rf;
@@ -785,7 +784,7 @@ __global__ void k_shade(PTArgs a, PTState S, int cur, int k)
             }
             T = T * weight;
             if (k >= sp.rr_start) T = T * (1.0f / sp.rr_prob);
-            S.T[p] = f4(T);
+            S.T[p] = make_float4(T.x, T.y, T.z, mt.type == MCRT_DIFF ? 1.0f : 0.0f);
             store_ray(S.rays[nxt], cslot, phit, nd, kEpsRay, __builtin_inff());
             S.act[nxt][cslot] = p;
             if (BATCH) S.rsid[nxt][cslot] = sid;
@@ -948,7 +947,6 @@ PTState state_of(StreamRes* r, bool adjoint)
     PTState S;
     S.T = w.T.as<float4>();
     S.L = w.L.as<float4>();
-    S.last_diff = w.flags.as<uint8_t>();
     for (int i = 0; i < 2; i++)
     {
         S.rays[i] = w.rays[i].as<float4>();
@@ -995,7 +993,6 @@ int pt_workspace(StreamRes* r, int64_t P, bool adjoint, bool batch, int rs)
     const size_t n = (size_t)P, nr = (size_t)(1 + rs) * n;
     if ((rc = w.T.ensure(n * 16))) return rc;
     if ((rc = w.L.ensure(n * 16))) return rc;
-    if ((rc = w.flags.ensure(n))) return rc;
     for (int i = 0; i < 2; i++)
     {
         if ((rc = w.rays[i].ensure(nr * 32))) return rc;
